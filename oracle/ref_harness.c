@@ -1,0 +1,98 @@
+/*
+ * ref_harness.c -- drives the reference C (compiled from /root/reference by
+ * oracle/Makefile into oracle/_ref/libdtfe_ref.so) from the shared sample
+ * stream.  TEST INFRASTRUCTURE ONLY.
+ *
+ * The reference draws its Monte-Carlo sample positions and its perturbation
+ * corners from libc rand() (src/dtfe.c:621-622 and :223).  This file defines
+ * rand() inside the reference library (linked -Bsymbolic, so the reference's
+ * calls bind here) and answers each call from the counter-based stream of
+ * oracle/dtfe_oracle.c, keyed by (seed, pixel, sample, draw):
+ *
+ *   - a call whose return address lies inside compute_density is a sample
+ *     draw; in a single-threaded run the reference visits pixels in 4x4-tile
+ *     order (src/dtfe.c:607-614) and draws x then y per sample, so the
+ *     harness replays that order to name the pixel and sample;
+ *   - any other call comes from perturb_ray and takes the next perturbation
+ *     draw (index 2, 3, ...) of the current sample.
+ *
+ * For direct calls of shoot_3d_ray, refh_set_ray() names the sample.
+ */
+#define _GNU_SOURCE
+#include <dlfcn.h>
+#include <link.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "dtfe_oracle.h"
+
+/* The reference's own compute_density, in the same shared object. */
+extern void compute_density(double *, int, int *, int, int, double, double, double *, float,
+                            double, double, double, const int, const double);
+
+static uintptr_t g_cd_lo, g_cd_hi;
+static uint64_t g_seed;
+static int g_samples = 1;
+static int64_t g_sample_draws;
+static int *g_visit; /* pixel index of the q-th pixel the reference visits */
+static int64_t g_npix;
+static uint64_t g_pixel;
+static uint32_t g_sample, g_draw = 2;
+
+static void refh_init_range(void) {
+  if (g_cd_hi) return;
+  Dl_info info;
+  const ElfW(Sym) *sym = NULL;
+  if (dladdr1((void *)&compute_density, &info, (void **)&sym, RTLD_DL_SYMENT) && sym) {
+    g_cd_lo = (uintptr_t)info.dli_saddr;
+    g_cd_hi = g_cd_lo + sym->st_size;
+  }
+}
+
+/* Map mode: the next compute_density call draws for a grid_dim^2 map with
+ * n_samp samples per pixel from stream `seed`. */
+void refh_set_map(uint64_t seed, int grid_dim, int n_samp) {
+  refh_init_range();
+  g_seed = seed;
+  g_samples = n_samp;
+  g_sample_draws = 0;
+  free(g_visit);
+  g_npix = (int64_t)grid_dim * grid_dim;
+  g_visit = (int *)malloc(sizeof(int) * (size_t)(g_npix > 0 ? g_npix : 1));
+  int64_t q = 0;
+  for (int band = 0; band < grid_dim / 4; ++band)
+    for (int tx = 0; tx < grid_dim / 4; ++tx)
+      for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 4; ++k) g_visit[q++] = (band * 4 + j) * grid_dim + tx * 4 + k;
+}
+
+/* Ray mode: perturbation draws of the next shoot_3d_ray call belong to
+ * (seed, pixel, sample). */
+void refh_set_ray(uint64_t seed, uint64_t pixel, uint32_t sample) {
+  refh_init_range();
+  g_seed = seed;
+  g_pixel = pixel;
+  g_sample = sample;
+  g_draw = 2;
+}
+
+int64_t refh_sample_draws(void) { return g_sample_draws; }
+uintptr_t refh_cd_size(void) {
+  refh_init_range();
+  return g_cd_hi - g_cd_lo;
+}
+
+int rand(void) {
+  uintptr_t ra = (uintptr_t)__builtin_return_address(0);
+  if (ra >= g_cd_lo && ra < g_cd_hi && g_visit) {
+    int64_t q = g_sample_draws++;
+    int64_t seq = q / 2;
+    uint32_t coord = (uint32_t)(q % 2);
+    int64_t visit = seq / g_samples;
+    g_sample = (uint32_t)(seq % g_samples);
+    g_pixel = visit < g_npix ? (uint64_t)g_visit[visit] : 0;
+    g_draw = 2;
+    return (int)sdo_draw31(g_seed, g_pixel, g_sample, coord);
+  }
+  return (int)sdo_draw31(g_seed, g_pixel, g_sample, g_draw++);
+}

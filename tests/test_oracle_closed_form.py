@@ -1,0 +1,272 @@
+"""Pin the oracle by mathematics the golden vectors leave open.
+
+Each check is chosen so that a plausible slip (a dropped term, a wrong sign,
+a transposed operand, a wrong neighbour slot) fails it:
+
+* volume of a tet with dyadic coordinates is exact; the tets tile the hull;
+* density of one tet is 4 m / V at every vertex (src/triangulate.c:223-224);
+* mass conservation: sum_t V_t * mean(rho at its vertices) = N m exactly
+  in exact arithmetic, since every vertex's volumes sum to its W_i;
+* gradients recover a linear field exactly (src/triangulate.c:233-260);
+* a ray through one tet integrates the linear field over the chord the tet
+  cuts from the line, computed here by clipping against the 4 face planes;
+* a ray through a whole tessellation of a linear field integrates that field
+  over the hull's z-extent at (x, y) (the midpoint rule is exact on lines);
+* entry lookup agrees with brute force over all forward-hull triangles;
+* the stream reproduces SplitMix64 as defined, including fixed values.
+"""
+import numpy as np
+import pytest
+from scipy.spatial import ConvexHull
+
+from sdtfe_amd.tessellate import orient_and_link, to_particle_data, triangulate
+
+MASK64 = (1 << 64) - 1
+
+
+def _mix64(z):
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+    return z ^ (z >> 31)
+
+
+def _draw31(seed, pixel, sample, draw):
+    g = 0x9E3779B97F4A7C15
+    k = _mix64((seed + g) & MASK64)
+    k = _mix64(k ^ pixel)
+    k = _mix64(k ^ sample)
+    return _mix64((k + (draw + 1) * g) & MASK64) >> 33
+
+
+def test_stream_definition(oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        s, p, r, d = (int(v) for v in rng.integers(0, 2**31, 4))
+        assert oracle.draw31(s, p, r, d) == _draw31(s, p, r, d)
+    # fixed values: a change of the stream changes every map
+    assert [oracle.draw31(0, 0, 0, d) for d in range(3)] == [_draw31(0, 0, 0, d) for d in range(3)]
+    assert _mix64(0) == 0 and _mix64(1) == 0x5692161D100B05E5
+    vals = np.array([oracle.draw31(1, p, 0, 0) for p in range(4000)])
+    assert vals.max() <= 2**31 - 1 and abs(vals.mean() / 2**31 - 0.5) < 0.02
+
+
+def _tet_case():
+    pos = np.array([[0.125, 0.25, 0.0625], [0.875, 0.25, 0.125], [0.25, 0.875, 0.25],
+                    [0.5, 0.375, 0.875]])
+    return pos, orient_and_link(pos, np.array([[0, 1, 2, 3]]), -np.ones((1, 4), dtype=int))
+
+
+def test_single_tet_volume_and_density(oracle):
+    pos, tetra = _tet_case()
+    pd = to_particle_data(pos)
+    a, b, c, d = pos
+    v_exact = abs(np.dot(a - d, np.cross(b - d, c - d))) / 6  # dyadic inputs: exact
+    assert oracle.tet_volumes(pd, tetra)[0] == v_exact
+    rho = oracle.density(pd, tetra, 0.5)
+    np.testing.assert_array_equal(rho, np.full(4, 2.0 / v_exact))
+    # orientation convention of src/triangulate.c:72-99: det(AB, AC, AD) <= 0
+    A, B, C, D = pos[tetra[0, :4]]
+    assert np.dot(np.cross(B - A, C - A), D - A) <= 0
+
+
+@pytest.mark.parametrize("n,seed", [(64, 0), (2000, 1)])
+def test_tessellation_invariants(n, seed):
+    pos = np.random.default_rng(seed).random((n, 3))
+    t = triangulate(pos)
+    P = pos[t[:, :4]]
+    det = np.einsum("ij,ij->i", np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), P[:, 3] - P[:, 0])
+    assert (det <= 0).all()
+    # slot 4+f is the neighbour across the face opposite vertex f
+    for f in range(4):
+        has = t[:, 4 + f] >= 0
+        me = t[has, :4]
+        other = t[t[has, 4 + f], :4]
+        face = np.delete(me, f, axis=1)
+        for k in range(3):
+            assert (other == face[:, k:k + 1]).any(axis=1).all()
+        assert not (other == me[:, f:f + 1]).any(axis=1).any()
+        back = t[t[has, 4 + f], 4:]
+        assert (back == np.nonzero(has)[0][:, None]).any(axis=1).all()
+    vol = np.abs(det) / 6
+    assert np.isclose(vol.sum(), ConvexHull(pos).volume, rtol=1e-12)
+    assert 5.5 < t.shape[0] / n < 7.0 or n < 500
+
+
+def test_empty_circumsphere_small():
+    pos = np.random.default_rng(5).random((60, 3))
+    t = triangulate(pos)
+    for row in t:
+        p = pos[row[:4]]
+        A = 2 * (p[1:] - p[0])
+        rhs = (p[1:] ** 2).sum(1) - (p[0] ** 2).sum()
+        c = np.linalg.solve(A, rhs)
+        r2 = ((p[0] - c) ** 2).sum()
+        d2 = ((pos - c) ** 2).sum(1)
+        d2[row[:4]] = np.inf
+        assert (d2 >= r2 * (1 - 1e-9)).all()
+
+
+def test_mass_conservation(oracle):
+    pos = np.random.default_rng(2).random((3000, 3))
+    t = triangulate(pos)
+    pd = to_particle_data(pos)
+    vol = oracle.tet_volumes(pd, t)
+    rho = oracle.density(pd, t, 0.625)
+    mass = (vol * rho[t[:, :4]].mean(axis=1)).sum()
+    assert mass == pytest.approx(3000 * 0.625, rel=1e-12)
+
+
+def test_gradients_recover_linear_field(oracle):
+    pos = np.random.default_rng(4).random((500, 3))
+    t = triangulate(pos)
+    pd = to_particle_data(pos)
+    coef = np.array([0.7, -1.3, 2.1])
+    pd[:, 3] = 3.0 + pos @ coef
+    nab = oracle.gradients(t, pd)
+    np.testing.assert_allclose(nab, np.broadcast_to(coef, nab.shape), rtol=0, atol=1e-7)
+    # and the gradient of a single tet with equal vertex values is zero
+    pos4, t4 = _tet_case()
+    pd4 = to_particle_data(pos4)
+    oracle.density(pd4, t4, 1.0)
+    assert np.array_equal(oracle.gradients(t4, pd4), np.zeros((1, 3)))
+
+
+def _chord(tet_pos, x, y):
+    """z-interval of the vertical line (x, y) inside the tet (half-spaces)."""
+    lo, hi = -np.inf, np.inf
+    c = tet_pos.mean(0)
+    for f in range(4):
+        p = np.delete(tet_pos, f, axis=0)
+        n = np.cross(p[1] - p[0], p[2] - p[0])
+        if np.dot(n, c - p[0]) < 0:
+            n = -n  # inward normal
+        # inside: n . (q - p0) >= 0 with q = (x, y, z)
+        a = n[2]
+        b = n[0] * (x - p[0, 0]) + n[1] * (y - p[0, 1]) - n[2] * p[0, 2]
+        if a > 0:
+            lo = max(lo, -b / a)
+        elif a < 0:
+            hi = min(hi, -b / a)
+        elif b < 0:
+            return None
+    return (lo, hi) if lo < hi else None
+
+
+def test_single_tet_chord_integral(oracle):
+    pos, tetra = _tet_case()
+    pd = to_particle_data(pos)
+    pd[:, 3] = [1.0, 2.0, 0.5, 3.0]  # a linear field through the 4 vertex values
+    nab = oracle.gradients(tetra, pd)
+    tri = oracle.triangles(tetra, pd)
+    rng = np.random.default_rng(9)
+    A = pd[tetra[0, 0]]
+    hits = 0
+    for _ in range(200):
+        x, y = rng.random(2)
+        ch = _chord(pos, x, y)
+        entry = oracle.pt_loc_2d(tri, pd, x, y)
+        v, c = oracle.shoot_ray(entry, tetra, nab, pd, x, y, -1.0, 3.0, 0.01, 0, 0, 0)
+        if ch is None:
+            assert entry == -1 and c == 0 and v == 0.0
+            continue
+        hits += 1
+        assert entry == 0 and c == 1
+        z0, z1 = ch
+        f = lambda z: A[3] + nab[0, 0] * (x - A[0]) + nab[0, 1] * (y - A[1]) + nab[0, 2] * (z - A[2])
+        exact = 0.5 * (f(z0) + f(z1)) * (z1 - z0)
+        assert v == pytest.approx(exact, rel=1e-11, abs=1e-14)
+    assert hits > 40
+
+
+def test_column_integral_of_linear_field(oracle):
+    pos = np.random.default_rng(6).random((1500, 3))
+    t = triangulate(pos)
+    pd = to_particle_data(pos)
+    coef = np.array([0.4, -0.9, 1.7])
+    pd[:, 3] = 2.0 + pos @ coef
+    nab = oracle.gradients(t, pd)
+    tri = oracle.triangles(t, pd)
+    hull = ConvexHull(pos)
+    rng = np.random.default_rng(7)
+    for _ in range(100):
+        x, y = 0.1 + 0.8 * rng.random(2)
+        # z-extent of the hull on the line: clip against hull facets
+        lo, hi = -np.inf, np.inf
+        for eq in hull.equations:  # n . q + d <= 0 inside
+            a = eq[2]
+            b = eq[0] * x + eq[1] * y + eq[3]
+            if a > 0:
+                hi = min(hi, -b / a)
+            elif a < 0:
+                lo = max(lo, -b / a)
+        entry = oracle.pt_loc_2d(tri, pd, x, y)
+        v, c = oracle.shoot_ray(entry, t, nab, pd, x, y, -1.0, 3.0, 0.01, 0, 0, 0)
+        assert c > 0
+        f = lambda z: 2.0 + coef[0] * x + coef[1] * y + coef[2] * z
+        exact = 0.5 * (f(lo) + f(hi)) * (hi - lo)
+        assert v == pytest.approx(exact, rel=1e-9)
+
+
+def test_entry_lookup_brute_force(oracle):
+    pos = np.random.default_rng(8).random((4000, 3))
+    t = triangulate(pos)
+    pd = to_particle_data(pos)
+    tri = oracle.triangles(t, pd)
+    P = pos[tri[:, :3], :2]
+    rng = np.random.default_rng(10)
+    q = rng.random((2000, 2)) * 1.1 - 0.05
+    for x, y in q:
+        e = oracle.pt_loc_2d(tri, pd, x, y)
+        d = []
+        for k in range(3):
+            a, b, o = P[:, k], P[:, (k + 1) % 3], P[:, (k + 2) % 3]
+            ins = (o[:, 0] - a[:, 0]) * (b[:, 1] - a[:, 1]) - (o[:, 1] - a[:, 1]) * (b[:, 0] - a[:, 0])
+            qry = (x - a[:, 0]) * (b[:, 1] - a[:, 1]) - (y - a[:, 1]) * (b[:, 0] - a[:, 0])
+            d.append(ins * qry >= 0)
+        inside = np.nonzero(d[0] & d[1] & d[2])[0]
+        if inside.size == 0:
+            assert e == -1
+        else:
+            assert e in tri[inside, 6]
+    # the forward faces tile the hull's shadow: their projected areas add up
+    a2 = np.abs((P[:, 1, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1])
+                - (P[:, 1, 1] - P[:, 0, 1]) * (P[:, 2, 0] - P[:, 0, 0])).sum() / 2
+    assert a2 == pytest.approx(ConvexHull(pos[:, :2]).volume, rel=1e-12)
+
+
+def test_map_mass_recovers_total(oracle):
+    """Sum over pixels of rho * pixel area estimates the total mass when the
+    hull's shadow is the map and the slab covers the depth.  (Pixels only
+    partly inside the shadow would be biased high: the reference averages
+    over the samples that hit, src/dtfe.c:626-642.)"""
+    corners = np.array([[i, j, k] for i in (0, 1) for j in (0, 1) for k in (0, 1)], float)
+    pos = np.concatenate([corners, np.random.default_rng(12).random((3000, 3)) * 0.98 + 0.01])
+    t = triangulate(pos)
+    pd = to_particle_data(pos)
+    G = 64
+    rho, st = oracle.compute_density(pd, t, G, 1.0, 2.0, 0.5, (0.5, 0.5, 0.5), 8, 0.0, 1)
+    total = rho.sum() * (1.0 / G) ** 2
+    assert total == pytest.approx(pos.shape[0] * 0.5, rel=0.005)
+    assert st["missed"] == 0 and st["failed"] == 0
+
+
+def test_failures_are_counted(oracle):
+    """Host-forced failure paths (src/dtfe.c:321-348, 626-636)."""
+    pos = np.random.default_rng(13).random((800, 3))
+    t = triangulate(pos)
+    pd = to_particle_data(pos)
+    oracle.density(pd, t, 1.0)
+    nab = oracle.gradients(t, pd)
+    tri = oracle.triangles(t, pd)
+    # outside the hull's shadow: the lookup misses
+    assert oracle.pt_loc_2d(tri, pd, 1.5, 0.5) == -1
+    # a neighbour pointing at a tet the ray does not cross ends the ray
+    x, y = 0.43, 0.57
+    entry = oracle.pt_loc_2d(tri, pd, x, y)
+    bad = t.copy()
+    far = int(np.argmax(((pos[t[:, 0], :2] - [x, y]) ** 2).sum(1)))
+    bad[:, 4:][bad[:, 4:] >= 0] = far
+    bad[far, 4:] = -1
+    st = oracle.Stats()
+    v, c = oracle.shoot_ray(entry, bad, nab, pd, x, y, -1.0, 3.0, 0.01, 0, 0, 0, st)
+    assert c == 0 and v == 0.0 and st.no_intersection == 1

@@ -1,0 +1,96 @@
+"""Pin the CPU oracle to golden vectors written by the reference C.
+
+The fixtures come from tests/golden/make_golden.py, which runs the
+reference's own pre_compute_vol / gradients / find_forward_hull /
+copy_fh_triangles / pt_loc_2d / shoot_3d_ray / compute_density
+(src/triangulate.c:183-395, src/dtfe.c:28-654).  Integer and index results
+must match exactly; floating-point results are compared bit for bit, since the
+oracle keeps the reference's operation order.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from sdtfe_amd import workloads
+from sdtfe_amd.tessellate import to_particle_data, triangulate
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("case", ["tet4", "pts8", "pts64", "dyadic"])
+def test_field_and_triangles_match_reference(oracle, golden, case):
+    g = golden("g1_tiny.npz")
+    pos, tetra = g[f"{case}_pos"], g[f"{case}_tetra"]
+    pd = to_particle_data(pos)
+    rho = oracle.density(pd, tetra, 0.75)
+    np.testing.assert_array_equal(rho, g[f"{case}_rho"])
+    nabla = oracle.gradients(tetra, pd)
+    np.testing.assert_array_equal(nabla, g[f"{case}_nabla"])
+    tri = oracle.triangles(tetra, pd)
+    np.testing.assert_array_equal(tri, g[f"{case}_tri"])
+
+
+def test_entry_lookup_and_rays_match_reference(oracle, golden):
+    g = golden("g2_rays.npz")
+    pd = to_particle_data(g["pos"])
+    pd[:, 3] = g["rho"]
+    tetra, nabla, tri = g["tetra"], g["nabla"], g["tri"]
+    seed, dxy, off_z, lz = int(g["seed"]), float(g["dxy"]), float(g["off_z"]), float(g["lz"])
+    n_c = int(g["n_crafted"])
+    st_plain, st_craft = oracle.Stats(), oracle.Stats()
+    n = g["x"].size
+    for i in range(n):
+        x, y = float(g["x"][i]), float(g["y"][i])
+        entry = oracle.pt_loc_2d(tri, pd, x, y)
+        assert entry == g["entry"][i], i
+        st = st_craft if i >= n - n_c else st_plain
+        v, c = oracle.shoot_ray(entry, tetra, nabla, pd, x, y, off_z, lz, dxy, seed,
+                                int(g["pixel"][i]), int(g["sample"][i]), st)
+        assert c == g["crossed"][i], i
+        assert v == g["value"][i], (i, v, g["value"][i])
+    # the crafted rays through vertices / along edges must exercise the
+    # perturbation (src/dtfe.c:329-349), otherwise the fixture pins nothing
+    assert st_craft.perturb_steps > 100
+    assert st_plain.samples == 0 and st_plain.tet_steps == 0  # shoot_ray does not count
+
+
+@pytest.mark.parametrize("name,G,S,L,Lz,cen,ds,m,seed",
+                         [pytest.param(*m, id=m[0]) for m in __import__(
+                             "tests.golden.make_golden", fromlist=["MAPS"]).MAPS])
+def test_maps_match_reference(oracle, golden, name, G, S, L, Lz, cen, ds, m, seed):
+    g = golden("g3_maps.npz")
+    pd = to_particle_data(g["pos"])
+    rho, st = oracle.compute_density(pd, g["tetra"], G, L, Lz, m, cen, S, ds, seed, n_threads=2)
+    np.testing.assert_array_equal(pd[:, 3], g[f"{name}_vertex_rho"])
+    np.testing.assert_array_equal(rho, g[f"{name}_rho"])
+    assert st["samples"] == G * G * S
+
+
+def test_uniform_32768_map_matches_reference(oracle, golden):
+    g = golden("g4_c1.npz")
+    pos = workloads.uniform(32768, 0)
+    assert _sha(pos) == str(g["pos_sha"])
+    tetra = triangulate(pos)
+    if _sha(tetra) != str(g["tetra_sha"]):
+        pytest.skip("this scipy/Qhull build tessellates differently from the fixture's")
+    spec = workloads.C1["spec"]
+    pd = to_particle_data(pos)
+    rho, st = oracle.compute_density(pd, tetra, spec.grid_dim, spec.box_len, spec.box_len_z,
+                                     spec.p_mass, spec.center, spec.n_samples,
+                                     spec.delta_sample, int(g["seed"]))
+    np.testing.assert_array_equal(rho, g["rho"])
+    assert st["samples"] == 256 * 256 * 4
+    assert 140 < st["tet_steps"] / (st["samples"] - st["missed"]) < 150
+
+
+def test_rho_accumulates(oracle, golden):
+    """rho is added to, not overwritten (src/dtfe.c:642)."""
+    g = golden("g3_maps.npz")
+    pd = to_particle_data(g["pos"])
+    base = np.full(64 * 64, 0.25)
+    rho, _ = oracle.compute_density(pd, g["tetra"], 64, 1.0, 1.0, 1.0, (0.5, 0.5, 0.5), 4,
+                                    0.0, 3, rho=base.copy())
+    np.testing.assert_array_equal(rho, base + g["plain_rho"])
