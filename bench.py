@@ -1,0 +1,235 @@
+"""Surface-density map throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config clustered|uniform|lattice]
+
+A step is one map of the workload with the tessellation resident in HBM:
+Monte-Carlo sample-rays are placed, located on the forward hull and walked
+through the tets, and pixel values are accumulated in HBM.  Default workload:
+1,048,576 clustered particles (30% uniform + 200 NFW-like halos, seed 0),
+2048x2048 pixels, 16 samples per pixel.  Particles are synthetic; the
+tessellation (host Qhull) and the upload are outside the timed region.
+
+Prints one JSON line: sample-rays/s of the whole job (value), the walk
+kernel's roofline (algorithmic bytes per tet crossed / kernel time from HIP
+events on the library's stream), and the CPU oracle timed on the host cores
+over a bounded subset of the same map (every k-th 4-row band).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic bytes per tet crossed: the tet row (8 x int32 = 32 B), one new
+# vertex position (3 x FP64 = 24 B; the other three are shared with the
+# previous tet) and the field (rho_A + gradient, 4 x FP64 = 32 B).
+BYTES_PER_STEP = 88
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "clustered": dict(n=1_048_576, grid=2048, samples=16, box_len=1.0, box_len_z=1.0,
+                      center=(0.5, 0.5, 0.5)),
+    "uniform": dict(n=32768, grid=256, samples=4, box_len=1.0, box_len_z=1.0,
+                    center=(0.5, 0.5, 0.5)),
+    "lattice": dict(n=8_388_608, grid=4096, samples=8, box_len=1.0, box_len_z=0.5,
+                    center=(0.5, 0.5, 0.25)),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def particles(name):
+    from sdtfe_amd import workloads
+    if name == "clustered":
+        return workloads.clustered(CONFIGS[name]["n"], seed=0)
+    if name == "uniform":
+        return workloads.uniform(CONFIGS[name]["n"], seed=0)
+    return workloads.jittered_lattice(seed=0)
+
+
+def rel_err(a, b):
+    m = np.abs(b) > 0
+    d = np.abs(a - b)[m] / np.abs(b)[m]
+    return d
+
+
+def time_form(tess, cfg, precision, steps, warmup, seed):
+    from sdtfe_amd import capi
+    G, S = cfg["grid"], cfg["samples"]
+    buf = capi.DeviceBuffer(G * G * 8)
+    p = tess.params(G, S, box_len=cfg["box_len"], box_len_z=cfg["box_len_z"],
+                    center=cfg["center"], seed=seed, precision=precision)
+    buf.zero()
+    for _ in range(warmup):
+        tess.map_device(p, buf, stats=True)
+    buf.zero()
+    tess.sync()
+    walk_ms, steps_per_map, st_last = [], [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = tess.map_device(p, buf, stats=True)
+        walk_ms.append(st["t_walk_ms"])
+        steps_per_map.append(st["tet_steps"])
+        st_last = st
+    tess.sync()
+    wall = time.perf_counter() - t0
+    # the accumulated map over `steps` identical maps, divided back
+    acc = buf.to_host() / steps
+    buf.close()
+    return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, walk_ms=float(np.mean(walk_ms)),
+                tet_steps=int(np.mean(steps_per_map)), stats=st_last, map=acc)
+
+
+def cpu_baseline(pos, tetra, cfg, seed, target_s, gpu_map):
+    """Oracle (C, OpenMP) on every k-th 4-row band of the same map."""
+    from oracle import oracle as O
+    from sdtfe_amd.tessellate import to_particle_data
+    if not os.path.exists(O.LIB_PATH):
+        O.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pd = to_particle_data(pos)
+    O.density(pd, tetra, 1.0)
+    nabla = O.gradients(tetra, pd)
+    tri = O.triangles(tetra, pd)
+    G, S = cfg["grid"], cfg["samples"]
+    bands = G // 4
+    args = (pd, tetra, nabla, tri, G, cfg["box_len"], cfg["box_len_z"], cfg["center"], S, 0.0,
+            seed)
+    # calibrate on one band in the middle, then take every k-th band
+    t0 = time.perf_counter()
+    O.map_bands(*args, band_begin=bands // 2, band_step=bands, n_threads=threads)
+    t_band = max(time.perf_counter() - t0, 1e-6)
+    k = max(1, int(np.ceil(bands * t_band / target_s)))
+    rho = np.zeros(G * G)
+    t0 = time.perf_counter()
+    _, rays, st = O.map_bands(*args, band_begin=k // 2, band_step=k, n_threads=threads, rho=rho)
+    t = time.perf_counter() - t0
+    rows = np.zeros(G, dtype=bool)
+    for b in range(k // 2, bands, k):
+        rows[4 * b:4 * b + 4] = True
+    sel = np.zeros((G, G), dtype=bool)
+    sel[rows, :] = True
+    sel = sel.ravel()
+    err = rel_err(gpu_map[sel], rho[sel]) if gpu_map is not None else np.zeros(1)
+    return dict(value=rays / t, unit="sample-rays/s", cores=threads, kind="port",
+                sample=(f"every {k}th 4-row band of the {G}x{G} map ({rays} sample-rays, "
+                        f"{100.0 * rays / (G * G * S):.2f}% of the map), same tessellation "
+                        f"and sample stream, {t:.1f} s"),
+                seconds=t, steps_per_ray=st["tet_steps"] / max(1, st["samples"] - st["missed"]),
+                gpu_vs_cpu_max_rel_err=float(err.max()) if err.size else 0.0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="clustered", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the path shards by "
+              "whole maps or 4-row bands, which this build does not run across GPUs.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from sdtfe_amd import capi
+    from sdtfe_amd.tessellate import to_particle_data, triangulate
+    cfg = CONFIGS[a.config]
+    G, S = cfg["grid"], cfg["samples"]
+    log(f"[bench] generating {a.config} particles")
+    pos = particles(a.config)
+    t0 = time.perf_counter()
+    tetra = triangulate(pos)
+    log(f"[bench] tessellated {pos.shape[0]} particles -> {tetra.shape[0]} tets in "
+        f"{time.perf_counter() - t0:.1f} s")
+    assert capi.device_count() >= 1, "no HIP device"
+    tess = capi.Tessellation(to_particle_data(pos), tetra)
+    info = tess.info()
+    log(f"[bench] resident: {info['device_bytes'] / 2**30:.2f} GiB, setup {info['t_setup_ms']:.0f} ms,"
+        f" {info['n_tri']} hull faces, grid {info['grid_nx']}x{info['grid_ny']}")
+
+    forms = {}
+    want = ["fp64", "mixed"] if a.precision == "auto" else [a.precision]
+    for name in want:
+        prec = capi.PREC_MIXED if name == "mixed" else capi.PREC_FP64
+        try:
+            log(f"[bench] {name}: {a.warmup} warmup + {a.steps} timed maps")
+            forms[name] = time_form(tess, cfg, prec, a.steps, a.warmup, a.seed)
+            f = forms[name]
+            log(f"[bench] {name}: {f['ms_per_step']:.1f} ms/map, walk {f['walk_ms']:.1f} ms, "
+                f"{f['tet_steps'] / (G * G * S):.1f} tets/ray")
+        except capi.SdtfeError as e:
+            log(f"[bench] {name}: unavailable ({e})")
+    tess.close()
+
+    err = None
+    if "fp64" in forms and "mixed" in forms:
+        e = rel_err(forms["mixed"]["map"], forms["fp64"]["map"])
+        err = dict(max=float(e.max()), p99=float(np.quantile(e, 0.99)),
+                   median=float(np.median(e)))
+        log(f"[bench] mixed vs fp64 per-pixel rel err: {err}")
+    # headline form: mixed when it meets 1e-5 against the FP64 map, else FP64
+    head = "mixed" if (err is not None and err["max"] <= 1e-5) else "fp64"
+    if head not in forms:
+        head = next(iter(forms))
+    f = forms[head]
+    rays = G * G * S
+    value = rays / (f["ms_per_step"] / 1e3)
+    achieved = f["tet_steps"] * BYTES_PER_STEP / (f["walk_ms"] / 1e3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel=f"map_{head}_kernel", bytes_per_step=BYTES_PER_STEP,
+                    tet_steps_per_launch=f["tet_steps"], walk_ms=round(f["walk_ms"], 3))
+
+    cpu = None
+    if not a.no_cpu:
+        log(f"[bench] CPU baseline (oracle, ~{a.cpu_seconds:.0f} s)")
+        cpu = cpu_baseline(pos, tetra, cfg, a.seed, a.cpu_seconds,
+                           forms["fp64"]["map"] if "fp64" in forms else None)
+        log(f"[bench] CPU {cpu['value']:.3e} sample-rays/s on {cpu['cores']} threads; "
+            f"GPU/CPU {value / cpu['value']:.1f}x")
+
+    out = {
+        "metric": "sample-rays/s", "value": round(value, 1), "unit": "sample-rays/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(f["ms_per_step"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{a.config}-{pos.shape[0]} {G}x{G} map, {S} samples/pixel",
+                   "particles": int(pos.shape[0]), "tets": int(tetra.shape[0]),
+                   "grid": G, "samples_per_pixel": S, "sample_rays": rays,
+                   "precision": head},
+        "roofline": roofline,
+        "cpu_baseline": None if cpu is None else {
+            k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "forms": {k: {"sample_rays_per_s": round(rays / (v["ms_per_step"] / 1e3), 1),
+                      "ms_per_step": round(v["ms_per_step"], 3),
+                      "walk_ms": round(v["walk_ms"], 3),
+                      "tet_steps": v["tet_steps"],
+                      "missed": v["stats"]["missed"], "failed": v["stats"]["failed"],
+                      "perturb_steps": v["stats"]["perturb_steps"],
+                      "fp64_redo": v["stats"]["fp64_redo"]} for k, v in forms.items()},
+        "mixed_vs_fp64_rel_err": err,
+    }
+    if cpu is not None:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 2)
+        out["fp64_vs_cpu_max_rel_err"] = cpu["gpu_vs_cpu_max_rel_err"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+/*
+ * sdtfe_amd.h -- C ABI of the MI355X (gfx950) DTFE surface-density library
+ * (libsdtfe_amd.so).  Plain pointers and sizes only.
+ *
+ * Drop-in entry points (same names, prototypes and argument meaning as the
+ * reference's headers, so existing C callers and the pydtfe binding link
+ * against this library unchanged):
+ *
+ *   compute_density   replaces src/dtfe.h:13-15 (src/dtfe.c:554-654)
+ *   pre_compute_vol   replaces src/triangulate.h:26 (src/triangulate.c:183-225)
+ *   gradients         replaces src/triangulate.h:27 (src/triangulate.c:233-260)
+ *
+ * Behaviour kept from the reference: particle_data is N x 4 float64 AoS
+ * (x, y, z, w) and column 3 is overwritten with the vertex density;
+ * tetra_data is T x 8 int32 rows (A, B, C, D, N_BDC, N_ACD, N_ADB, N_ABC);
+ * rho is grid_dim^2 float64, row-major rho[y * grid_dim + x], and is
+ * ACCUMULATED into (+=); grid_dim must be a multiple of 4; samples that miss
+ * the hull's shadow or whose walk fails are dropped from the pixel average.
+ *
+ * Differences, by design: Monte-Carlo sample positions come from a
+ * counter-based stream keyed by (seed, pixel, sample) instead of libc rand(),
+ * so maps are reproducible and independent of scheduling; failures are
+ * counted (sdtfe_stats) instead of printed per sample; a walk is cut after
+ * SDTFE_MAX_STEPS_DEFAULT tets (the reference has no bound) and counted.
+ *
+ * The extended API keeps a tessellation resident in HBM across maps
+ * (sdtfe_tess_*), offers the faster mixed-precision walk, and reports
+ * counters and kernel times.  Functions returning int return 0 on success
+ * and a negative SDTFE_E* code on failure; sdtfe_last_error() describes the
+ * last failure of the calling thread.
+ */
+#ifndef SDTFE_AMD_H
+#define SDTFE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------- drop-in */
+
+/* src/dtfe.h:13-15.  Runs on the current HIP device in FP64 (results match
+ * the reference path fed the same sample positions).  Sample stream seed:
+ * sdtfe_set_seed(); each call uses the current seed and then advances it, so
+ * repeated calls draw fresh samples as the reference's rand() would. */
+void compute_density(double *particle_data, int n_particles, int *tetra_data, int n_tetra,
+                     int grid_dim, double box_len, double box_len_z, double *rho,
+                     float p_mass, double center_x, double center_y, double center_z,
+                     const int n_mc_samp, const double delta_sample);
+
+/* src/triangulate.h:26: writes rho_i = 4 m / W_i into (*particle_data)[:,3]. */
+void pre_compute_vol(double **particle_data, int n_particles, int *tetra_data, int n_tetra,
+                     float p_mass);
+
+/* src/triangulate.h:27: returns a malloc'd n_tetra x 3 array (free() it),
+ * the per-tet gradient of the vertex values in particle_data[:,3]. */
+double *gradients(int *tetra_data, int n_tetra, double *particle_data);
+
+/* ---------------------------------------------------------- extended */
+
+enum {
+  SDTFE_OK = 0,
+  SDTFE_EINVAL = -1,   /* bad argument */
+  SDTFE_EHIP = -2,     /* HIP runtime error */
+  SDTFE_ENOMEM = -3,   /* device allocation failed */
+  SDTFE_ETOPO = -4,    /* tessellation indices out of range */
+  SDTFE_ENODEV = -5    /* no HIP device */
+};
+
+enum { SDTFE_PREC_FP64 = 0, SDTFE_PREC_MIXED = 1 };
+
+#define SDTFE_MAX_STEPS_DEFAULT (1 << 24)
+
+typedef struct sdtfe_tess sdtfe_tess; /* device-resident tessellation */
+
+typedef struct {
+  int grid_dim;        /* pixels per side, multiple of 4 */
+  int n_mc_samp;       /* Monte-Carlo samples per pixel */
+  double box_len;      /* map width and height */
+  double box_len_z;    /* slab depth */
+  double center[3];
+  double delta_sample; /* MC sampling width (pixel width if smaller) */
+  uint64_t seed;       /* sample stream seed */
+  int precision;       /* SDTFE_PREC_* */
+  int max_steps;       /* per-ray tet bound, <= 0 for the default */
+} sdtfe_map_params;
+
+typedef struct {
+  int64_t samples;           /* sample-rays requested */
+  int64_t missed;            /* outside the hull's shadow */
+  int64_t failed;            /* rays that ended with 0 tets crossed */
+  int64_t no_intersection;   /* src/dtfe.c:321-325 */
+  int64_t perturb_exhausted; /* src/dtfe.c:338-342 */
+  int64_t no_isect_perturb;  /* src/dtfe.c:344-348 */
+  int64_t perturb_steps;     /* perturbations taken */
+  int64_t tet_steps;         /* tets crossed by all rays */
+  int64_t step_cap;          /* rays cut by max_steps */
+  int64_t fp64_redo;         /* mixed walk: edge products redone in FP64 */
+  double t_walk_ms;          /* walk kernel time (HIP events) */
+  double t_total_ms;         /* whole call, host wall clock */
+} sdtfe_stats;
+
+typedef struct {
+  int n_particles, n_tetra, n_tri; /* n_tri: forward-hull faces */
+  int grid_nx, grid_ny;            /* entry-lookup grid */
+  int64_t device_bytes;            /* HBM held by the tessellation */
+  double t_setup_ms;               /* upload + field + entry structure */
+} sdtfe_tess_info;
+
+const char *sdtfe_last_error(void);
+void sdtfe_set_seed(uint64_t seed);
+uint64_t sdtfe_get_seed(void);
+
+/* Upload particles (N x 4, column 3 ignored) and tets once, then compute on
+ * the GPU: tet volumes, vertex densities (4 m / W, W summed in ascending tet
+ * order), gradients, packed walk records and the entry-lookup grid. */
+int sdtfe_tess_create(const double *particle_data, int n_particles, const int *tetra_data,
+                      int n_tetra, float p_mass, sdtfe_tess **out);
+/* Same, but the vertex field is taken from particle_data[:,3] as given
+ * (no density pass): the route by which the reference interpolates an
+ * arbitrary particle field (python/pydtfe.cpp:156-158). */
+int sdtfe_tess_create_field(const double *particle_data, int n_particles,
+                            const int *tetra_data, int n_tetra, sdtfe_tess **out);
+void sdtfe_tess_destroy(sdtfe_tess *t);
+int sdtfe_tess_get_info(const sdtfe_tess *t, sdtfe_tess_info *info);
+/* Copies out: vertex densities (N), gradients (T x 3), forward-hull
+ * triangles as (a, b, c, -1, -1, -1, tet, 0) rows in the reference's order
+ * (neighbour slots are not built: the grid lookup does not walk them). */
+int sdtfe_tess_vertex_density(const sdtfe_tess *t, double *rho);
+int sdtfe_tess_gradients(const sdtfe_tess *t, double *nabla);
+int sdtfe_tess_triangles(const sdtfe_tess *t, int *tri_rows, int capacity);
+
+/* Entry tet of each query point (-1 outside the hull's shadow). */
+int sdtfe_tess_locate(sdtfe_tess *t, int n, const double *x, const double *y, int *entry);
+
+/* Walk n explicit rays (host arrays).  entry may be NULL (then located);
+ * (pixel, sample) key the perturbation draws, which start at draw 2. */
+int sdtfe_tess_trace(sdtfe_tess *t, int precision, int n, const double *x, const double *y,
+                     const int *entry, const uint64_t *pixel, const uint32_t *sample,
+                     double off_z, double lz, double delta_xy, uint64_t seed,
+                     double *value, int *crossed, sdtfe_stats *stats);
+
+/* One map.  rho (host, grid_dim^2) is accumulated into; n_avg (host,
+ * nullable) receives each pixel's count of contributing samples. */
+int sdtfe_map(sdtfe_tess *t, const sdtfe_map_params *p, double *rho, int *n_avg,
+              sdtfe_stats *stats);
+
+/* One map into device memory (accumulated into), asynchronous on the
+ * tessellation's stream; with stats != NULL it waits and fills them. */
+int sdtfe_map_device(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_device,
+                     sdtfe_stats *stats);
+int sdtfe_tess_sync(sdtfe_tess *t);
+
+/* Device buffers for callers without another HIP allocator. */
+void *sdtfe_device_alloc(size_t bytes);
+int sdtfe_device_free(void *p);
+int sdtfe_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int sdtfe_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int sdtfe_memset_d(void *p, int value, size_t bytes);
+int sdtfe_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,282 @@
+"""ctypes binding of libsdtfe_amd.so (the C ABI in include/sdtfe_amd.h).
+
+The library is the product path; there is no CPU fallback.  Loading fails
+loudly if the library has not been built (``python -m sdtfe_amd.build``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsdtfe_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "sdtfe_amd.h")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+MAX_STEPS_DEFAULT = 1 << 24
+
+
+class SdtfeError(RuntimeError):
+    pass
+
+
+class MapParams(C.Structure):
+    _fields_ = [("grid_dim", C.c_int), ("n_mc_samp", C.c_int), ("box_len", C.c_double),
+                ("box_len_z", C.c_double), ("center", C.c_double * 3),
+                ("delta_sample", C.c_double), ("seed", C.c_uint64), ("precision", C.c_int),
+                ("max_steps", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "samples", "missed", "failed", "no_intersection", "perturb_exhausted",
+        "no_isect_perturb", "perturb_steps", "tet_steps", "step_cap", "fp64_redo")] + [
+        ("t_walk_ms", C.c_double), ("t_total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class TessInfo(C.Structure):
+    _fields_ = [("n_particles", C.c_int), ("n_tetra", C.c_int), ("n_tri", C.c_int),
+                ("grid_nx", C.c_int), ("grid_ny", C.c_int), ("device_bytes", C.c_int64),
+                ("t_setup_ms", C.c_double)]
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+_SIGS = {
+    "sdtfe_last_error": (C.c_char_p, []),
+    "sdtfe_set_seed": (None, [C.c_uint64]),
+    "sdtfe_get_seed": (C.c_uint64, []),
+    "sdtfe_tess_create": (C.c_int, [_dp, C.c_int, _ip, C.c_int, C.c_float, C.POINTER(_vp)]),
+    "sdtfe_tess_create_field": (C.c_int, [_dp, C.c_int, _ip, C.c_int, C.POINTER(_vp)]),
+    "sdtfe_tess_destroy": (None, [_vp]),
+    "sdtfe_tess_get_info": (C.c_int, [_vp, C.POINTER(TessInfo)]),
+    "sdtfe_tess_vertex_density": (C.c_int, [_vp, _dp]),
+    "sdtfe_tess_gradients": (C.c_int, [_vp, _dp]),
+    "sdtfe_tess_triangles": (C.c_int, [_vp, _ip, C.c_int]),
+    "sdtfe_tess_locate": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip]),
+    "sdtfe_tess_trace": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _vp, _u64p, _u32p,
+                                   C.c_double, C.c_double, C.c_double, C.c_uint64, _dp, _ip,
+                                   C.POINTER(Stats)]),
+    "sdtfe_map": (C.c_int, [_vp, C.POINTER(MapParams), _dp, _vp, C.POINTER(Stats)]),
+    "sdtfe_map_device": (C.c_int, [_vp, C.POINTER(MapParams), _vp, C.POINTER(Stats)]),
+    "sdtfe_tess_sync": (C.c_int, [_vp]),
+    "sdtfe_device_alloc": (_vp, [C.c_size_t]),
+    "sdtfe_device_free": (C.c_int, [_vp]),
+    "sdtfe_memcpy_d2h": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "sdtfe_memcpy_h2d": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "sdtfe_memset_d": (C.c_int, [_vp, C.c_int, C.c_size_t]),
+    "sdtfe_device_count": (C.c_int, []),
+    "compute_density": (None, [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double, C.c_double,
+                               _dp, C.c_float, C.c_double, C.c_double, C.c_double, C.c_int,
+                               C.c_double]),
+    "pre_compute_vol": (None, [C.POINTER(_vp), C.c_int, _ip, C.c_int, C.c_float]),
+    "gradients": (_vp, [_ip, C.c_int, _dp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libsdtfe_amd.so; raise if it is missing (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SdtfeError(f"{LIB_PATH} not built; run `python -m sdtfe_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def header_functions() -> list[str]:
+    """Function names declared in include/sdtfe_amd.h."""
+    import re
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", txt)))
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise SdtfeError(f"sdtfe error {rc}: {lib().sdtfe_last_error().decode()}")
+
+
+def device_count() -> int:
+    return int(lib().sdtfe_device_count())
+
+
+def _c64(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class DeviceBuffer:
+    """A device allocation owned by Python (freed on close/GC)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().sdtfe_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise SdtfeError(lib().sdtfe_last_error().decode())
+
+    def zero(self):
+        check(lib().sdtfe_memset_d(self.ptr, 0, self.nbytes))
+
+    def to_host(self, dtype=np.float64, count=None):
+        n = self.nbytes // np.dtype(dtype).itemsize if count is None else count
+        out = np.empty(n, dtype=dtype)
+        check(lib().sdtfe_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().sdtfe_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tessellation:
+    """A tessellation resident in HBM (sdtfe_tess)."""
+
+    def __init__(self, particle_data: np.ndarray, tetra_data: np.ndarray, p_mass: float = 1.0,
+                 given_field: bool = False):
+        pd = _c64(particle_data)
+        if pd.ndim != 2 or pd.shape[1] != 4:
+            raise ValueError("particle_data must be N x 4 float64")
+        tet = _c64(tetra_data, np.int32)
+        if tet.ndim != 2 or tet.shape[1] != 8:
+            raise ValueError("tetra_data must be T x 8 int32")
+        h = C.c_void_p()
+        if given_field:
+            rc = lib().sdtfe_tess_create_field(pd, pd.shape[0], tet, tet.shape[0], C.byref(h))
+        else:
+            rc = lib().sdtfe_tess_create(pd, pd.shape[0], tet, tet.shape[0], p_mass, C.byref(h))
+        check(rc)
+        self.h = h
+        self.n = pd.shape[0]
+        self.nt = tet.shape[0]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sdtfe_tess_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self) -> dict:
+        i = TessInfo()
+        check(lib().sdtfe_tess_get_info(self.h, C.byref(i)))
+        return {n: getattr(i, n) for n, _ in i._fields_}
+
+    def vertex_density(self) -> np.ndarray:
+        out = np.empty(self.n)
+        check(lib().sdtfe_tess_vertex_density(self.h, out))
+        return out
+
+    def gradients(self) -> np.ndarray:
+        out = np.empty((self.nt, 3))
+        check(lib().sdtfe_tess_gradients(self.h, out))
+        return out
+
+    def triangles(self) -> np.ndarray:
+        f = self.info()["n_tri"]
+        out = np.zeros((max(f, 1), 8), dtype=np.int32)
+        check(lib().sdtfe_tess_triangles(self.h, out, f))
+        return out[:f]
+
+    def locate(self, x, y) -> np.ndarray:
+        x, y = _c64(x), _c64(y)
+        out = np.empty(x.size, dtype=np.int32)
+        check(lib().sdtfe_tess_locate(self.h, x.size, x, y, out))
+        return out
+
+    def trace(self, x, y, pixel, sample, off_z, lz, delta_xy, seed, entry=None,
+              precision=PREC_FP64):
+        x, y = _c64(x), _c64(y)
+        n = x.size
+        pix = _c64(pixel, np.uint64)
+        smp = _c64(sample, np.uint32)
+        val = np.empty(n)
+        crossed = np.empty(n, dtype=np.int32)
+        ent = None if entry is None else _c64(entry, np.int32)
+        st = Stats()
+        check(lib().sdtfe_tess_trace(self.h, precision, n, x, y,
+                                     None if ent is None else ent.ctypes.data, pix, smp,
+                                     off_z, lz, delta_xy, seed, val, crossed, C.byref(st)))
+        return val, crossed, st.as_dict()
+
+    def params(self, grid_dim, n_samples, box_len=1.0, box_len_z=1.0, center=(0.5, 0.5, 0.5),
+               delta_sample=0.0, seed=0, precision=PREC_FP64, max_steps=0) -> MapParams:
+        p = MapParams()
+        p.grid_dim, p.n_mc_samp = int(grid_dim), int(n_samples)
+        p.box_len, p.box_len_z = float(box_len), float(box_len_z)
+        p.center[:] = [float(c) for c in center]
+        p.delta_sample, p.seed = float(delta_sample), int(seed)
+        p.precision, p.max_steps = int(precision), int(max_steps)
+        return p
+
+    def map(self, grid_dim, n_samples, rho=None, want_navg=False, **kw):
+        """One map on the GPU; returns (rho, stats[, n_avg]).  rho is
+        accumulated into if given."""
+        p = self.params(grid_dim, n_samples, **kw)
+        npix = grid_dim * grid_dim
+        out = np.zeros(npix) if rho is None else rho
+        if out.dtype != np.float64 or not out.flags.c_contiguous or out.size != npix:
+            raise ValueError("rho must be a contiguous float64 array of grid_dim^2")
+        navg = np.empty(npix, dtype=np.int32) if want_navg else None
+        st = Stats()
+        check(lib().sdtfe_map(self.h, C.byref(p), out,
+                              None if navg is None else navg.ctypes.data, C.byref(st)))
+        return (out, st.as_dict(), navg) if want_navg else (out, st.as_dict())
+
+    def map_device(self, params: MapParams, rho_dev: DeviceBuffer, stats: bool = True):
+        st = Stats() if stats else None
+        check(lib().sdtfe_map_device(self.h, C.byref(params), rho_dev.ptr,
+                                     C.byref(st) if st is not None else None))
+        return st.as_dict() if st is not None else None
+
+    def sync(self):
+        check(lib().sdtfe_tess_sync(self.h))
+
+
+def compute_density(particle_data, tetra_data, grid_dim, box_len, box_len_z, rho, p_mass,
+                    center, n_mc_samp, delta_sample) -> None:
+    """The drop-in C entry point (src/dtfe.h:13-15) called through the ABI:
+    particle_data[:, 3] is overwritten and rho accumulated in place."""
+    if particle_data.dtype != np.float64 or not particle_data.flags.c_contiguous:
+        raise ValueError("particle_data must be C-contiguous float64")
+    tet = _c64(tetra_data, np.int32)
+    lib().compute_density(particle_data, particle_data.shape[0], tet, tet.shape[0], grid_dim,
+                          box_len, box_len_z, rho, p_mass, center[0], center[1], center[2],
+                          n_mc_samp, delta_sample)
+
+
+def set_seed(seed: int) -> None:
+    lib().sdtfe_set_seed(int(seed))

@@ -1,0 +1,486 @@
+// capi.cpp -- extern "C" entry points of libsdtfe_amd.so (declared in
+// include/sdtfe_amd.h).  Host C++ over the HIP runtime; no torch types.
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sdtfe_host.h"
+
+namespace sdtfe {
+
+static thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return 0;
+  return fail(SDTFE_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate);
+void launch_trace_fp64(sdtfe_tess *t, const MapGeom &g, int n, const double *x, const double *y,
+                       const int *entry, const uint64_t *pixel, const uint32_t *sample,
+                       double *value, int *crossed);
+void launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate);
+void launch_trace_mixed(sdtfe_tess *t, const MapGeom &g, int n, const double *x,
+                        const double *y, const int *entry, const uint64_t *pixel,
+                        const uint32_t *sample, double *value, int *crossed);
+void launch_locate(sdtfe_tess *t, int n, const double *x, const double *y, int *entry);
+
+extern const bool kHaveMixed;
+
+namespace {
+
+std::atomic<uint64_t> g_seed{0};
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+// Map geometry exactly as src/dtfe.c:579-589 computes it.
+MapGeom make_geom(const sdtfe_tess *t, const sdtfe_map_params &p) {
+  MapGeom g{};
+  g.dxy = p.box_len / p.grid_dim;
+  g.off_x = -(p.box_len / 2.0) + p.center[0];
+  g.off_y = -(p.box_len / 2.0) + p.center[1];
+  g.off_z = -(p.box_len_z / 2.0) + p.center[2];
+  g.lz = p.box_len_z;
+  g.scale = p.delta_sample > g.dxy ? p.delta_sample : g.dxy;
+  g.soff = p.delta_sample > g.dxy ? (p.delta_sample - g.dxy) / 2.0 : 0.0;
+  g.org_x = t->org[0];
+  g.org_y = t->org[1];
+  g.org_z = t->org[2];
+  g.seed = p.seed;
+  g.grid = p.grid_dim;
+  g.samples = p.n_mc_samp;
+  g.max_steps = p.max_steps > 0 ? p.max_steps : SDTFE_MAX_STEPS_DEFAULT;
+  return g;
+}
+
+int check_params(const sdtfe_map_params *p) {
+  if (!p) return fail(SDTFE_EINVAL, "null map parameters");
+  if (p->grid_dim <= 0 || p->grid_dim > 65536) return fail(SDTFE_EINVAL, "grid_dim out of range");
+  if (p->n_mc_samp < 0) return fail(SDTFE_EINVAL, "n_mc_samp < 0");
+  if (!(p->box_len > 0) || !(p->box_len_z > 0)) return fail(SDTFE_EINVAL, "box sizes must be > 0");
+  if (p->precision != SDTFE_PREC_FP64 && p->precision != SDTFE_PREC_MIXED)
+    return fail(SDTFE_EINVAL, "unknown precision");
+  if (p->precision == SDTFE_PREC_MIXED && !kHaveMixed)
+    return fail(SDTFE_EINVAL, "mixed-precision walk not built");
+  return 0;
+}
+
+int ensure_map_buf(sdtfe_tess *t, size_t npix) {
+  if (npix <= t->map_cap) return 0;
+  if (t->map_buf) (void)hipFree(t->map_buf);
+  if (t->navg_buf) (void)hipFree(t->navg_buf);
+  t->map_buf = nullptr;
+  t->navg_buf = nullptr;
+  t->map_cap = 0;
+  if (hipMalloc((void **)&t->map_buf, sizeof(double) * npix) != hipSuccess ||
+      hipMalloc((void **)&t->navg_buf, sizeof(int) * npix) != hipSuccess)
+    return fail(SDTFE_ENOMEM, "map buffer allocation failed");
+  t->map_cap = npix;
+  return 0;
+}
+
+void fill_stats(const unsigned long long *c, sdtfe_stats *s) {
+  s->samples = (int64_t)c[ST_SAMPLES];
+  s->missed = (int64_t)c[ST_MISSED];
+  s->failed = (int64_t)c[ST_FAILED];
+  s->no_intersection = (int64_t)c[ST_NO_ISECT];
+  s->perturb_exhausted = (int64_t)c[ST_PERT_EXHAUSTED];
+  s->no_isect_perturb = (int64_t)c[ST_NO_ISECT_PERT];
+  s->perturb_steps = (int64_t)c[ST_PERT_STEPS];
+  s->tet_steps = (int64_t)c[ST_TET_STEPS];
+  s->step_cap = (int64_t)c[ST_STEP_CAP];
+  s->fp64_redo = (int64_t)c[ST_FP64_REDO];
+}
+
+int read_stats(sdtfe_tess *t, sdtfe_stats *s) {
+  unsigned long long c[ST_COUNT];
+  SDTFE_HIP(hipMemcpyAsync(c, t->stats, sizeof c, hipMemcpyDeviceToHost, t->stream));
+  SDTFE_HIP(hipStreamSynchronize(t->stream));
+  if (s) fill_stats(c, s);
+  return 0;
+}
+
+template <typename T>
+struct DevArray {
+  T *p = nullptr;
+  ~DevArray() {
+    if (p) (void)hipFree(p);
+  }
+  int upload(const T *h, size_t n, hipStream_t s) {
+    if (hipMalloc((void **)&p, sizeof(T) * (n ? n : 1)) != hipSuccess)
+      return fail(SDTFE_ENOMEM, "scratch allocation failed");
+    if (h && n) SDTFE_HIP(hipMemcpyAsync(p, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    return 0;
+  }
+};
+
+void free_tess(sdtfe_tess *t) {
+  if (!t) return;
+  void *bufs[] = {t->pos4,     t->tets,    t->vol,      t->nabla,    t->rec64,
+                  t->rec32,    t->fld32,   t->tri_xy,   t->tri_tet,  t->tri_abc,
+                  t->cell_off, t->cell_tri, t->stats,   t->map_buf,  t->navg_buf};
+  if (t->stream) (void)hipStreamSynchronize(t->stream);
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (t->ev0) (void)hipEventDestroy(t->ev0);
+  if (t->ev1) (void)hipEventDestroy(t->ev1);
+  if (t->stream) (void)hipStreamDestroy(t->stream);
+  delete t;
+}
+
+int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool given_field,
+           sdtfe_tess **out) {
+  if (!out) return fail(SDTFE_EINVAL, "null output handle");
+  *out = nullptr;
+  if (n < 0 || nt < 0) return fail(SDTFE_EINVAL, "negative size");
+  if ((n > 0 && !pd) || (nt > 0 && !tetra)) return fail(SDTFE_EINVAL, "null input array");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(SDTFE_ENODEV, "no HIP device visible");
+  auto t0 = Clock::now();
+  sdtfe_tess *t = new sdtfe_tess();
+  t->n = n;
+  t->nt = nt;
+  t->p_mass = p_mass;
+  t->given_field = given_field;
+  int rc = hip_check(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking), "stream");
+  if (!rc) rc = hip_check(hipEventCreate(&t->ev0), "event");
+  if (!rc) rc = hip_check(hipEventCreate(&t->ev1), "event");
+  if (!rc && hipMalloc((void **)&t->stats, sizeof(unsigned long long) * ST_COUNT) != hipSuccess)
+    rc = fail(SDTFE_ENOMEM, "stats allocation failed");
+  if (!rc) rc = build_field(t, pd, tetra);
+  if (!rc) rc = build_entry_grid(t);
+  if (rc) {
+    free_tess(t);
+    return rc;
+  }
+  t->t_setup_ms = ms_since(t0);
+  *out = t;
+  return 0;
+}
+
+int run_map(sdtfe_tess *t, const sdtfe_map_params *p, double *out, int *navg, int accumulate,
+            double *t_walk_ms) {
+  const MapGeom g = make_geom(t, *p);
+  SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
+  SDTFE_HIP(hipEventRecord(t->ev0, t->stream));
+  if (p->precision == SDTFE_PREC_MIXED)
+    launch_map_mixed(t, g, out, navg, accumulate);
+  else
+    launch_map_fp64(t, g, out, navg, accumulate);
+  SDTFE_HIP(hipGetLastError());
+  SDTFE_HIP(hipEventRecord(t->ev1, t->stream));
+  if (t_walk_ms) {
+    SDTFE_HIP(hipEventSynchronize(t->ev1));
+    float ms = 0;
+    SDTFE_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    *t_walk_ms = ms;
+  }
+  return 0;
+}
+
+}  // namespace
+}  // namespace sdtfe
+
+using namespace sdtfe;
+
+extern "C" {
+
+const char *sdtfe_last_error(void) { return g_err.c_str(); }
+void sdtfe_set_seed(uint64_t seed) { g_seed.store(seed); }
+uint64_t sdtfe_get_seed(void) { return g_seed.load(); }
+
+int sdtfe_tess_create(const double *particle_data, int n_particles, const int *tetra_data,
+                      int n_tetra, float p_mass, sdtfe_tess **out) {
+  return create(particle_data, n_particles, tetra_data, n_tetra, p_mass, false, out);
+}
+
+int sdtfe_tess_create_field(const double *particle_data, int n_particles, const int *tetra_data,
+                            int n_tetra, sdtfe_tess **out) {
+  return create(particle_data, n_particles, tetra_data, n_tetra, 1.0f, true, out);
+}
+
+void sdtfe_tess_destroy(sdtfe_tess *t) { free_tess(t); }
+
+int sdtfe_tess_get_info(const sdtfe_tess *t, sdtfe_tess_info *info) {
+  if (!t || !info) return fail(SDTFE_EINVAL, "null argument");
+  info->n_particles = t->n;
+  info->n_tetra = t->nt;
+  info->n_tri = t->n_tri;
+  info->grid_nx = t->grid.nx;
+  info->grid_ny = t->grid.ny;
+  info->device_bytes = t->device_bytes;
+  info->t_setup_ms = t->t_setup_ms;
+  return 0;
+}
+
+int sdtfe_tess_vertex_density(const sdtfe_tess *t, double *rho) {
+  if (!t || (!rho && t->n)) return fail(SDTFE_EINVAL, "null argument");
+  if (t->n == 0) return 0;
+  std::vector<double> tmp(4 * (size_t)t->n);
+  SDTFE_HIP(hipMemcpyAsync(tmp.data(), t->pos4, sizeof(double) * tmp.size(),
+                           hipMemcpyDeviceToHost, t->stream));
+  SDTFE_HIP(hipStreamSynchronize(t->stream));
+  for (int i = 0; i < t->n; ++i) rho[i] = tmp[4 * (size_t)i + 3];
+  return 0;
+}
+
+int sdtfe_tess_gradients(const sdtfe_tess *t, double *nabla) {
+  if (!t || (!nabla && t->nt)) return fail(SDTFE_EINVAL, "null argument");
+  if (t->nt == 0) return 0;
+  SDTFE_HIP(hipMemcpyAsync(nabla, t->nabla, sizeof(double) * 3 * (size_t)t->nt,
+                           hipMemcpyDeviceToHost, t->stream));
+  SDTFE_HIP(hipStreamSynchronize(t->stream));
+  return 0;
+}
+
+int sdtfe_tess_triangles(const sdtfe_tess *t, int *rows, int capacity) {
+  if (!t) return fail(SDTFE_EINVAL, "null tessellation");
+  if (capacity < t->n_tri || (!rows && t->n_tri)) return fail(SDTFE_EINVAL, "capacity too small");
+  const int f = t->n_tri;
+  if (f == 0) return 0;
+  std::vector<int> abc(3 * (size_t)f), tet((size_t)f);
+  SDTFE_HIP(hipMemcpyAsync(abc.data(), t->tri_abc, sizeof(int) * abc.size(),
+                           hipMemcpyDeviceToHost, t->stream));
+  SDTFE_HIP(hipMemcpyAsync(tet.data(), t->tri_tet, sizeof(int) * tet.size(),
+                           hipMemcpyDeviceToHost, t->stream));
+  SDTFE_HIP(hipStreamSynchronize(t->stream));
+  for (int k = 0; k < f; ++k) {
+    int *r = rows + 8 * (size_t)k;
+    r[0] = abc[3 * (size_t)k];
+    r[1] = abc[3 * (size_t)k + 1];
+    r[2] = abc[3 * (size_t)k + 2];
+    r[3] = r[4] = r[5] = -1;
+    r[6] = tet[k];
+    r[7] = 0;
+  }
+  return 0;
+}
+
+int sdtfe_tess_locate(sdtfe_tess *t, int n, const double *x, const double *y, int *entry) {
+  if (!t || n < 0 || (n && (!x || !y || !entry))) return fail(SDTFE_EINVAL, "bad argument");
+  if (n == 0) return 0;
+  DevArray<double> dx, dy;
+  DevArray<int> de;
+  int rc;
+  if ((rc = dx.upload(x, n, t->stream)) || (rc = dy.upload(y, n, t->stream)) ||
+      (rc = de.upload(nullptr, n, t->stream)))
+    return rc;
+  launch_locate(t, n, dx.p, dy.p, de.p);
+  SDTFE_HIP(hipGetLastError());
+  SDTFE_HIP(hipMemcpyAsync(entry, de.p, sizeof(int) * n, hipMemcpyDeviceToHost, t->stream));
+  SDTFE_HIP(hipStreamSynchronize(t->stream));
+  return 0;
+}
+
+int sdtfe_tess_trace(sdtfe_tess *t, int precision, int n, const double *x, const double *y,
+                     const int *entry, const uint64_t *pixel, const uint32_t *sample,
+                     double off_z, double lz, double delta_xy, uint64_t seed, double *value,
+                     int *crossed, sdtfe_stats *stats) {
+  if (!t || n < 0 || (n && (!x || !y || !pixel || !sample || !value || !crossed)))
+    return fail(SDTFE_EINVAL, "bad argument");
+  if (precision != SDTFE_PREC_FP64 && precision != SDTFE_PREC_MIXED)
+    return fail(SDTFE_EINVAL, "unknown precision");
+  auto t0 = Clock::now();
+  MapGeom g{};
+  g.dxy = delta_xy;
+  g.off_z = off_z;
+  g.lz = lz;
+  g.org_x = t->org[0];
+  g.org_y = t->org[1];
+  g.org_z = t->org[2];
+  g.seed = seed;
+  g.max_steps = SDTFE_MAX_STEPS_DEFAULT;
+  SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
+  if (n > 0) {
+    DevArray<double> dx, dy, dv;
+    DevArray<int> de, dc;
+    DevArray<uint64_t> dp;
+    DevArray<uint32_t> ds;
+    int rc;
+    if ((rc = dx.upload(x, n, t->stream)) || (rc = dy.upload(y, n, t->stream)) ||
+        (rc = dp.upload(pixel, n, t->stream)) || (rc = ds.upload(sample, n, t->stream)) ||
+        (rc = dv.upload(nullptr, n, t->stream)) || (rc = dc.upload(nullptr, n, t->stream)))
+      return rc;
+    if (entry && (rc = de.upload(entry, n, t->stream))) return rc;
+    if (precision == SDTFE_PREC_MIXED)
+      launch_trace_mixed(t, g, n, dx.p, dy.p, entry ? de.p : nullptr, dp.p, ds.p, dv.p, dc.p);
+    else
+      launch_trace_fp64(t, g, n, dx.p, dy.p, entry ? de.p : nullptr, dp.p, ds.p, dv.p, dc.p);
+    SDTFE_HIP(hipGetLastError());
+    SDTFE_HIP(hipMemcpyAsync(value, dv.p, sizeof(double) * n, hipMemcpyDeviceToHost, t->stream));
+    SDTFE_HIP(hipMemcpyAsync(crossed, dc.p, sizeof(int) * n, hipMemcpyDeviceToHost, t->stream));
+    SDTFE_HIP(hipStreamSynchronize(t->stream));
+  }
+  int rc = read_stats(t, stats);
+  if (stats) stats->t_total_ms = ms_since(t0);
+  return rc;
+}
+
+int sdtfe_map(sdtfe_tess *t, const sdtfe_map_params *p, double *rho, int *n_avg,
+              sdtfe_stats *stats) {
+  if (!t) return fail(SDTFE_EINVAL, "null tessellation");
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!rho) return fail(SDTFE_EINVAL, "null rho");
+  auto t0 = Clock::now();
+  const size_t npix = (size_t)p->grid_dim * (size_t)p->grid_dim;
+  if ((rc = ensure_map_buf(t, npix))) return rc;
+  double t_walk = 0;
+  if ((rc = run_map(t, p, t->map_buf, t->navg_buf, 0, &t_walk))) return rc;
+  std::vector<double> val(npix);
+  SDTFE_HIP(hipMemcpyAsync(val.data(), t->map_buf, sizeof(double) * npix, hipMemcpyDeviceToHost,
+                           t->stream));
+  if (n_avg)
+    SDTFE_HIP(hipMemcpyAsync(n_avg, t->navg_buf, sizeof(int) * npix, hipMemcpyDeviceToHost,
+                             t->stream));
+  SDTFE_HIP(hipStreamSynchronize(t->stream));
+  for (size_t i = 0; i < npix; ++i) rho[i] += val[i];  // src/dtfe.c:642-645
+  if ((rc = read_stats(t, stats))) return rc;
+  if (stats) {
+    stats->t_walk_ms = t_walk;
+    stats->t_total_ms = ms_since(t0);
+  }
+  return 0;
+}
+
+int sdtfe_map_device(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_device,
+                     sdtfe_stats *stats) {
+  if (!t) return fail(SDTFE_EINVAL, "null tessellation");
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!rho_device) return fail(SDTFE_EINVAL, "null rho");
+  auto t0 = Clock::now();
+  double t_walk = 0;
+  if ((rc = run_map(t, p, rho_device, nullptr, 1, stats ? &t_walk : nullptr))) return rc;
+  if (stats) {
+    if ((rc = read_stats(t, stats))) return rc;
+    stats->t_walk_ms = t_walk;
+    stats->t_total_ms = ms_since(t0);
+  }
+  return 0;
+}
+
+int sdtfe_tess_sync(sdtfe_tess *t) {
+  if (!t) return fail(SDTFE_EINVAL, "null tessellation");
+  SDTFE_HIP(hipStreamSynchronize(t->stream));
+  return 0;
+}
+
+void *sdtfe_device_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+    fail(SDTFE_ENOMEM, "hipMalloc failed");
+    return nullptr;
+  }
+  return p;
+}
+int sdtfe_device_free(void *p) {
+  SDTFE_HIP(hipFree(p));
+  return 0;
+}
+int sdtfe_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  SDTFE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+int sdtfe_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  SDTFE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+int sdtfe_memset_d(void *p, int value, size_t bytes) {
+  SDTFE_HIP(hipMemset(p, value, bytes));
+  return 0;
+}
+int sdtfe_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// ------------------------------------------------------------ drop-in
+
+static int dropin_precision() {
+  const char *e = std::getenv("SDTFE_PRECISION");
+  return (e && std::strcmp(e, "mixed") == 0) ? SDTFE_PREC_MIXED : SDTFE_PREC_FP64;
+}
+
+void compute_density(double *particle_data, int n_particles, int *tetra_data, int n_tetra,
+                     int grid_dim, double box_len, double box_len_z, double *rho, float p_mass,
+                     double center_x, double center_y, double center_z, const int n_mc_samp,
+                     const double delta_sample) {
+  sdtfe_tess *t = nullptr;
+  int rc = sdtfe_tess_create(particle_data, n_particles, tetra_data, n_tetra, p_mass, &t);
+  if (rc == 0) {
+    sdtfe_map_params p{};
+    p.grid_dim = grid_dim;
+    p.n_mc_samp = n_mc_samp;
+    p.box_len = box_len;
+    p.box_len_z = box_len_z;
+    p.center[0] = center_x;
+    p.center[1] = center_y;
+    p.center[2] = center_z;
+    p.delta_sample = delta_sample;
+    p.seed = g_seed.fetch_add(1);
+    p.precision = dropin_precision();
+    rc = sdtfe_map(t, &p, rho, nullptr, nullptr);
+    if (rc == 0 && n_particles > 0) {
+      std::vector<double> r((size_t)n_particles);
+      rc = sdtfe_tess_vertex_density(t, r.data());
+      if (rc == 0)
+        for (int i = 0; i < n_particles; ++i) particle_data[4 * (size_t)i + 3] = r[i];
+    }
+  }
+  if (rc) std::fprintf(stderr, "compute_density (sdtfe_amd): %s\n", g_err.c_str());
+  sdtfe_tess_destroy(t);
+}
+
+void pre_compute_vol(double **particle_data, int n_particles, int *tetra_data, int n_tetra,
+                     float p_mass) {
+  if (!particle_data) return;
+  sdtfe_tess *t = nullptr;
+  int rc = sdtfe_tess_create(*particle_data, n_particles, tetra_data, n_tetra, p_mass, &t);
+  if (rc == 0 && n_particles > 0) {
+    std::vector<double> r((size_t)n_particles);
+    rc = sdtfe_tess_vertex_density(t, r.data());
+    if (rc == 0)
+      for (int i = 0; i < n_particles; ++i) (*particle_data)[4 * (size_t)i + 3] = r[i];
+  }
+  if (rc) std::fprintf(stderr, "pre_compute_vol (sdtfe_amd): %s\n", g_err.c_str());
+  sdtfe_tess_destroy(t);
+}
+
+double *gradients(int *tetra_data, int n_tetra, double *particle_data) {
+  double *nabla = (double *)std::malloc(sizeof(double) * 3 * (size_t)(n_tetra > 0 ? n_tetra : 1));
+  if (!nabla) return nullptr;
+  // vertex count: the largest index the tets name
+  int n = 0;
+  for (int64_t i = 0; i < 4 * (int64_t)n_tetra; ++i) {
+    const int v = tetra_data[8 * (i / 4) + i % 4];
+    if (v + 1 > n) n = v + 1;
+  }
+  sdtfe_tess *t = nullptr;
+  int rc = sdtfe_tess_create_field(particle_data, n, tetra_data, n_tetra, &t);
+  if (rc == 0) rc = sdtfe_tess_gradients(t, nabla);
+  if (rc) {
+    std::fprintf(stderr, "gradients (sdtfe_amd): %s\n", g_err.c_str());
+    std::free(nabla);
+    nabla = nullptr;
+  }
+  sdtfe_tess_destroy(t);
+  return nabla;
+}
+
+}  // extern "C"

@@ -1,0 +1,537 @@
+// field.hip -- per-tessellation setup on the GPU (gfx950):
+//   validation of the uploaded tets, tet volumes, vertex -> tet CSR, vertex
+//   density rho_i = 4 m / W_i, per-tet gradients, packed walk records, and
+//   the forward-hull entry grid.
+//
+// Arithmetic that decides results is written in the reference's operation
+// order and built with -ffp-contract=off, so volumes, densities and
+// gradients equal the reference's bit for bit:
+//   volume    src/triangulate.c:190-215
+//   density   src/triangulate.c:217-224 (W summed in ascending tet order)
+//   gradient  src/triangulate.c:228-257
+//   forward   src/triangulate.c:274-311
+//   triangles src/triangulate.c:325-355 (order: tet, then face BDC..ABC)
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sdtfe_host.h"
+#include "walk_common.h"
+
+namespace sdtfe {
+
+// ============================================================ scan
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+__device__ inline int wave_inclusive(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int n = __shfl_up(v, off, 64);
+    if (lane >= off) v += n;
+  }
+  return v;
+}
+
+// Exclusive scan over the block; wsum holds blockDim/64 + 1 ints.
+__device__ inline int block_exclusive(int v, int *wsum, int *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int inc = wave_inclusive(v);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int i = 0; i < nw; ++i) {
+      int s = wsum[i];
+      wsum[i] = acc;
+      acc += s;
+    }
+    wsum[nw] = acc;
+  }
+  __syncthreads();
+  int ex = inc - v + wsum[w];
+  *total = wsum[nw];
+  __syncthreads();
+  return ex;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const int *in, int64_t n,
+                                                                    int *partial) {
+  __shared__ int wsum[kScanThreads / 64 + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) s += in[base + i];
+  int tot;
+  block_exclusive(s, wsum, &tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void scan_partials_kernel(int *partial, int nb, int *total) {
+  __shared__ int wsum[1024 / 64 + 1];
+  const int per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int b0 = threadIdx.x * per;
+  int s = 0;
+  for (int i = 0; i < per; ++i)
+    if (b0 + i < nb) s += partial[b0 + i];
+  int tot;
+  int ex = block_exclusive(s, wsum, &tot);
+  for (int i = 0; i < per; ++i)
+    if (b0 + i < nb) {
+      int v = partial[b0 + i];
+      partial[b0 + i] = ex;
+      ex += v;
+    }
+  if (threadIdx.x == 0) *total = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const int *in, int *out,
+                                                                   int64_t n,
+                                                                   const int *partial) {
+  __shared__ int wsum[kScanThreads / 64 + 1];
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = (base + i < n) ? in[base + i] : 0;
+    s += v[i];
+  }
+  int tot;
+  int ex = block_exclusive(s, wsum, &tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) out[base + i] = ex;
+    ex += v[i];
+  }
+}
+
+size_t scan_tmp_ints(int64_t n) { return (size_t)((n + kScanChunk - 1) / kScanChunk) + 1; }
+
+int scan_exclusive(const int *in, int *out, int64_t n, int *tmp, int *total_host,
+                   hipStream_t s) {
+  if (n <= 0) {
+    if (total_host) *total_host = 0;
+    return 0;
+  }
+  const int64_t nb = (n + kScanChunk - 1) / kScanChunk;
+  if (nb > 1024 * 64) return fail(SDTFE_EINVAL, "scan: too many items");
+  int *partial = tmp, *total = tmp + nb;
+  scan_reduce_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, partial);
+  scan_partials_kernel<<<1, 1024, 0, s>>>(partial, (int)nb, total);
+  scan_apply_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, out, n, partial);
+  SDTFE_HIP(hipGetLastError());
+  if (total_host) {
+    SDTFE_HIP(hipMemcpyAsync(total_host, total, sizeof(int), hipMemcpyDeviceToHost, s));
+    SDTFE_HIP(hipStreamSynchronize(s));
+  }
+  return 0;
+}
+
+// ============================================================ field
+
+__device__ inline int4 tet_verts(const int *tets, int t) {
+  return reinterpret_cast<const int4 *>(tets)[2 * (int64_t)t];
+}
+__device__ inline int4 tet_nbrs(const int *tets, int t) {
+  return reinterpret_cast<const int4 *>(tets)[2 * (int64_t)t + 1];
+}
+
+__global__ void validate_kernel(const int *tets, int nt, int n, int *bad) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  int4 v = tet_verts(tets, t), nb = tet_nbrs(tets, t);
+  bool ok = v.x >= 0 && v.x < n && v.y >= 0 && v.y < n && v.z >= 0 && v.z < n && v.w >= 0 &&
+            v.w < n && nb.x >= -1 && nb.x < nt && nb.y >= -1 && nb.y < nt && nb.z >= -1 &&
+            nb.z < nt && nb.w >= -1 && nb.w < nt;
+  if (!ok) atomicOr(bad, 1);
+}
+
+// (A - D) . ((B - D) x (C - D)) in the reference's expansion order.
+__device__ inline double vol6(const double4 &a, const double4 &b, const double4 &c,
+                              const double4 &d) {
+  double m0 = (b.y - d.y) * (c.z - d.z) - (b.z - d.z) * (c.y - d.y);
+  double m1 = (b.z - d.z) * (c.x - d.x) - (b.x - d.x) * (c.z - d.z);
+  double m2 = (b.x - d.x) * (c.y - d.y) - (b.y - d.y) * (c.x - d.x);
+  return (a.x - d.x) * m0 + (a.y - d.y) * m1 + (a.z - d.z) * m2;
+}
+
+__global__ void volume_degree_kernel(const double4 *pos, const int *tets, int nt, double *vol,
+                                     int *deg) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  int4 v = tet_verts(tets, t);
+  vol[t] = fabs(vol6(pos[v.x], pos[v.y], pos[v.z], pos[v.w]) / 6);
+  atomicAdd(deg + v.x, 1);
+  atomicAdd(deg + v.y, 1);
+  atomicAdd(deg + v.z, 1);
+  atomicAdd(deg + v.w, 1);
+}
+
+__global__ void fill_incidence_kernel(const int *tets, int nt, const int *off, int *cursor,
+                                      int *inc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  int4 v = tet_verts(tets, t);
+  inc[off[v.x] + atomicAdd(cursor + v.x, 1)] = t;
+  inc[off[v.y] + atomicAdd(cursor + v.y, 1)] = t;
+  inc[off[v.z] + atomicAdd(cursor + v.z, 1)] = t;
+  inc[off[v.w] + atomicAdd(cursor + v.w, 1)] = t;
+}
+
+// One vertex per thread: put its incident tets in ascending order (the
+// order the reference adds them) and sum their volumes.
+__global__ void vertex_density_kernel(double4 *pos, int n, const int *off, int *inc,
+                                      const double *vol, float m4) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int b = off[v], e = off[v + 1];
+  for (int i = b + 1; i < e; ++i) {
+    int key = inc[i];
+    int j = i - 1;
+    while (j >= b && inc[j] > key) {
+      inc[j + 1] = inc[j];
+      --j;
+    }
+    inc[j + 1] = key;
+  }
+  double w = 0;
+  for (int i = b; i < e; ++i) w += vol[inc[i]];
+  pos[v].w = (double)m4 / w;
+}
+
+__device__ inline double det3(double a, double b, double c, double d, double e, double f,
+                              double g, double h, double i) {
+  return a * e * i + b * f * g + c * d * h - c * e * g - b * d * i - a * f * h;
+}
+
+__global__ void gradient_record_kernel(const double4 *pos, const int *tets, int nt,
+                                       double *nabla, Rec64 *rec, Rec32 *rec32,
+                                       Field32 *fld, double ox, double oy, double oz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int4 v = tet_verts(tets, t), nb = tet_nbrs(tets, t);
+  const double4 a = pos[v.x], b = pos[v.y], c = pos[v.z], d = pos[v.w];
+  const double xb = b.x - a.x, yb = b.y - a.y, zb = b.z - a.z;
+  const double xc = c.x - a.x, yc = c.y - a.y, zc = c.z - a.z;
+  const double xd = d.x - a.x, yd = d.y - a.y, zd = d.z - a.z;
+  const double rb = b.w - a.w, rc = c.w - a.w, rd = d.w - a.w;
+  const double den = det3(xb, yb, zb, xc, yc, zc, xd, yd, zd);
+  const double gx = det3(rb, yb, zb, rc, yc, zc, rd, yd, zd) / den;
+  const double gy = det3(xb, rb, zb, xc, rc, zc, xd, rd, zd) / den;
+  const double gz = det3(xb, yb, rb, xc, yc, rc, xd, yd, rd) / den;
+  nabla[3 * (int64_t)t + 0] = gx;
+  nabla[3 * (int64_t)t + 1] = gy;
+  nabla[3 * (int64_t)t + 2] = gz;
+
+  int4 *ri = reinterpret_cast<int4 *>(rec + t);
+  ri[0] = nb;
+  double *rd64 = reinterpret_cast<double *>(ri + 1);
+  rd64[0] = a.x; rd64[1] = a.y; rd64[2] = a.z;
+  rd64[3] = b.x; rd64[4] = b.y; rd64[5] = b.z;
+  rd64[6] = c.x; rd64[7] = c.y; rd64[8] = c.z;
+  rd64[9] = d.x; rd64[10] = d.y; rd64[11] = d.z;
+  rd64[12] = a.w; rd64[13] = gx; rd64[14] = gy; rd64[15] = gz;
+
+  Rec32 q;
+  q.nb[0] = nb.x; q.nb[1] = nb.y; q.nb[2] = nb.z; q.nb[3] = nb.w;
+  const double4 *p4[4] = {&a, &b, &c, &d};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    q.v[3 * k + 0] = (float)(p4[k]->x - ox);
+    q.v[3 * k + 1] = (float)(p4[k]->y - oy);
+    q.v[3 * k + 2] = (float)(p4[k]->z - oz);
+  }
+  rec32[t] = q;
+  fld[t] = Field32{(float)a.w, (float)gx, (float)gy, (float)gz};
+}
+
+// ============================================================ entry grid
+
+// Face f of a tet as the triangle the reference emits (src/triangulate.c:
+// 328-343); the forward test uses (p1 - p0) x (p2 - p0) (:279-308).
+__device__ inline void face_triple(int f, int4 v, int &p0, int &p1, int &p2) {
+  switch (f) {
+    case 0: p0 = v.y; p1 = v.w; p2 = v.z; break;  // BDC
+    case 1: p0 = v.x; p1 = v.z; p2 = v.w; break;  // ACD
+    case 2: p0 = v.x; p1 = v.w; p2 = v.y; break;  // ADB
+    default: p0 = v.x; p1 = v.y; p2 = v.z; break; // ABC
+  }
+}
+
+__global__ void hull_mask_kernel(const double4 *pos, const int *tets, int nt,
+                                 unsigned char *mask, int *cnt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int4 v = tet_verts(tets, t), nb = tet_nbrs(tets, t);
+  const int nbf[4] = {nb.x, nb.y, nb.z, nb.w};
+  unsigned m = 0;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    if (nbf[f] != -1) continue;
+    int i0, i1, i2;
+    face_triple(f, v, i0, i1, i2);
+    const double4 p0 = pos[i0], p1 = pos[i1], p2 = pos[i2];
+    double t1x = p1.x - p0.x, t1y = p1.y - p0.y;
+    double t2x = p2.x - p0.x, t2y = p2.y - p0.y;
+    if (t1x * t2y - t1y * t2x < 0.0) m |= 1u << f;
+  }
+  mask[t] = (unsigned char)m;
+  cnt[t] = __popc(m);
+}
+
+__global__ void emit_tri_kernel(const double4 *pos, const int *tets, int nt,
+                                const unsigned char *mask, const int *off, double *tri_xy,
+                                int *tri_tet, int *tri_abc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const unsigned m = mask[t];
+  if (!m) return;
+  const int4 v = tet_verts(tets, t);
+  int k = off[t];
+  for (int f = 0; f < 4; ++f) {
+    if (!(m >> f & 1u)) continue;
+    int i0, i1, i2;
+    face_triple(f, v, i0, i1, i2);
+    const double4 p0 = pos[i0], p1 = pos[i1], p2 = pos[i2];
+    double *xy = tri_xy + 6 * (int64_t)k;
+    xy[0] = p0.x; xy[1] = p0.y; xy[2] = p1.x; xy[3] = p1.y; xy[4] = p2.x; xy[5] = p2.y;
+    tri_abc[3 * k + 0] = i0;
+    tri_abc[3 * k + 1] = i1;
+    tri_abc[3 * k + 2] = i2;
+    tri_tet[k] = t;
+    ++k;
+  }
+}
+
+__global__ __launch_bounds__(256) void tri_bbox_kernel(const double *tri_xy, int n_tri,
+                                                       double *out) {
+  __shared__ double s[4][256];
+  double lo_x = INFINITY, lo_y = INFINITY, hi_x = -INFINITY, hi_y = -INFINITY;
+  for (int k = threadIdx.x; k < n_tri; k += blockDim.x) {
+    const double *xy = tri_xy + 6 * (int64_t)k;
+    for (int j = 0; j < 3; ++j) {
+      lo_x = fmin(lo_x, xy[2 * j]);
+      hi_x = fmax(hi_x, xy[2 * j]);
+      lo_y = fmin(lo_y, xy[2 * j + 1]);
+      hi_y = fmax(hi_y, xy[2 * j + 1]);
+    }
+  }
+  s[0][threadIdx.x] = lo_x; s[1][threadIdx.x] = lo_y;
+  s[2][threadIdx.x] = hi_x; s[3][threadIdx.x] = hi_y;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s[0][threadIdx.x] = fmin(s[0][threadIdx.x], s[0][threadIdx.x + h]);
+      s[1][threadIdx.x] = fmin(s[1][threadIdx.x], s[1][threadIdx.x + h]);
+      s[2][threadIdx.x] = fmax(s[2][threadIdx.x], s[2][threadIdx.x + h]);
+      s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int j = 0; j < 4; ++j) out[j] = s[j][0];
+}
+
+template <bool kFill>
+__global__ void cell_bin_kernel(EntryGrid g, int *cnt_or_cursor, const int *off, int *cell_tri) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.n_tri) return;
+  const double *xy = g.tri_xy + 6 * (int64_t)k;
+  double lx = fmin(fmin(xy[0], xy[2]), xy[4]), hx = fmax(fmax(xy[0], xy[2]), xy[4]);
+  double ly = fmin(fmin(xy[1], xy[3]), xy[5]), hy = fmax(fmax(xy[1], xy[3]), xy[5]);
+  const int cx0 = grid_cell(lx, g.x0, g.sx, g.nx), cx1 = grid_cell(hx, g.x0, g.sx, g.nx);
+  const int cy0 = grid_cell(ly, g.y0, g.sy, g.ny), cy1 = grid_cell(hy, g.y0, g.sy, g.ny);
+  for (int cy = cy0; cy <= cy1; ++cy)
+    for (int cx = cx0; cx <= cx1; ++cx) {
+      const int c = cy * g.nx + cx;
+      if (kFill)
+        cell_tri[off[c] + atomicAdd(cnt_or_cursor + c, 1)] = k;
+      else
+        atomicAdd(cnt_or_cursor + c, 1);
+    }
+}
+
+// ============================================================ host side
+
+namespace {
+
+template <typename T>
+int dev_alloc(sdtfe_tess *t, T **p, size_t count) {
+  size_t bytes = sizeof(T) * (count > 0 ? count : 1);
+  hipError_t e = hipMalloc((void **)p, bytes);
+  if (e != hipSuccess) return fail(SDTFE_ENOMEM, "hipMalloc failed (" + std::to_string(bytes) +
+                                                     " B): " + hipGetErrorString(e));
+  t->device_bytes += (int64_t)bytes;
+  return 0;
+}
+
+struct Scratch {
+  std::vector<void *> ptrs;
+  ~Scratch() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  int get(T **p, size_t count) {
+    size_t bytes = sizeof(T) * (count > 0 ? count : 1);
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) return fail(SDTFE_ENOMEM, "hipMalloc (scratch) failed");
+    ptrs.push_back(*p);
+    return 0;
+  }
+};
+
+inline unsigned blocks_for(int64_t n, int threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+}  // namespace
+
+#define SDTFE_RC(expr)     \
+  do {                     \
+    int _rc = (expr);      \
+    if (_rc) return _rc;   \
+  } while (0)
+
+int build_field(sdtfe_tess *t, const double *pd, const int *tetra) {
+  const int n = t->n, nt = t->nt;
+  hipStream_t s = t->stream;
+  SDTFE_RC(dev_alloc(t, &t->pos4, 4 * (size_t)n));
+  SDTFE_RC(dev_alloc(t, &t->tets, 8 * (size_t)nt));
+  SDTFE_HIP(hipMemcpyAsync(t->pos4, pd, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice, s));
+  SDTFE_HIP(hipMemcpyAsync(t->tets, tetra, sizeof(int) * 8 * (size_t)nt, hipMemcpyHostToDevice, s));
+
+  Scratch sc;
+  int *bad = nullptr;
+  SDTFE_RC(sc.get(&bad, 1));
+  SDTFE_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  if (nt > 0) validate_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(t->tets, nt, n, bad);
+  int bad_h = 0;
+  SDTFE_HIP(hipMemcpyAsync(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  SDTFE_HIP(hipStreamSynchronize(s));
+  if (bad_h) return fail(SDTFE_ETOPO, "tetra_data holds an index out of range");
+
+  // volumes and the vertex -> tet incidence (CSR)
+  SDTFE_RC(dev_alloc(t, &t->vol, (size_t)nt));
+  int *deg = nullptr, *off = nullptr, *cursor = nullptr, *inc = nullptr, *tmp = nullptr;
+  SDTFE_RC(sc.get(&deg, (size_t)n + 1));
+  SDTFE_RC(sc.get(&off, (size_t)n + 1));
+  SDTFE_RC(sc.get(&cursor, (size_t)n + 1));
+  SDTFE_RC(sc.get(&inc, 4 * (size_t)nt));
+  SDTFE_RC(sc.get(&tmp, scan_tmp_ints((int64_t)std::max(n + 1, nt))));
+  SDTFE_HIP(hipMemsetAsync(deg, 0, sizeof(int) * ((size_t)n + 1), s));
+  SDTFE_HIP(hipMemsetAsync(cursor, 0, sizeof(int) * ((size_t)n + 1), s));
+  if (nt > 0)
+    volume_degree_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
+        reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->vol, deg);
+  SDTFE_RC(scan_exclusive(deg, off, (int64_t)n + 1, tmp, nullptr, s));
+  if (nt > 0)
+    fill_incidence_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(t->tets, nt, off, cursor, inc);
+  const float m4 = t->p_mass * 4;  // a float product, src/triangulate.c:224
+  if (n > 0 && !t->given_field)
+    vertex_density_kernel<<<blocks_for(n, 256), 256, 0, s>>>(
+        reinterpret_cast<double4 *>(t->pos4), n, off, inc, t->vol, m4);
+
+  // origin of the FP32 records: centre of the particles' bounding box
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], pd[4 * (size_t)i + k]);
+      hi[k] = std::max(hi[k], pd[4 * (size_t)i + k]);
+    }
+  t->span = 0;
+  for (int k = 0; k < 3; ++k) {
+    t->org[k] = n > 0 ? 0.5 * (lo[k] + hi[k]) : 0.0;
+    if (n > 0) t->span = std::max(t->span, 0.5 * (hi[k] - lo[k]));
+  }
+
+  SDTFE_RC(dev_alloc(t, &t->nabla, 3 * (size_t)nt));
+  SDTFE_RC(dev_alloc(t, &t->rec64, (size_t)nt));
+  SDTFE_RC(dev_alloc(t, &t->rec32, (size_t)nt));
+  SDTFE_RC(dev_alloc(t, &t->fld32, (size_t)nt));
+  if (nt > 0)
+    gradient_record_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
+        reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->nabla, t->rec64, t->rec32,
+        t->fld32, t->org[0], t->org[1], t->org[2]);
+  SDTFE_HIP(hipGetLastError());
+  SDTFE_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int build_entry_grid(sdtfe_tess *t) {
+  const int nt = t->nt;
+  hipStream_t s = t->stream;
+  Scratch sc;
+  unsigned char *mask = nullptr;
+  int *cnt = nullptr, *off = nullptr, *tmp = nullptr;
+  SDTFE_RC(sc.get(&mask, (size_t)nt));
+  SDTFE_RC(sc.get(&cnt, (size_t)nt));
+  SDTFE_RC(sc.get(&off, (size_t)nt));
+  SDTFE_RC(sc.get(&tmp, scan_tmp_ints(nt)));
+  if (nt > 0)
+    hull_mask_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
+        reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, mask, cnt);
+  int n_tri = 0;
+  SDTFE_RC(scan_exclusive(cnt, off, nt, tmp, &n_tri, s));
+  t->n_tri = n_tri;
+  SDTFE_RC(dev_alloc(t, &t->tri_xy, 6 * (size_t)n_tri));
+  SDTFE_RC(dev_alloc(t, &t->tri_tet, (size_t)n_tri));
+  SDTFE_RC(dev_alloc(t, &t->tri_abc, 3 * (size_t)n_tri));
+  if (nt > 0)
+    emit_tri_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
+        reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, mask, off, t->tri_xy,
+        t->tri_tet, t->tri_abc);
+
+  EntryGrid g{};
+  g.tri_xy = t->tri_xy;
+  g.tri_tet = t->tri_tet;
+  g.n_tri = n_tri;
+  g.nx = g.ny = 1;
+  double bb[4] = {0, 0, 0, 0};
+  if (n_tri > 0) {
+    double *bbd = nullptr;
+    SDTFE_RC(sc.get(&bbd, 4));
+    tri_bbox_kernel<<<1, 256, 0, s>>>(t->tri_xy, n_tri, bbd);
+    SDTFE_HIP(hipMemcpyAsync(bb, bbd, sizeof bb, hipMemcpyDeviceToHost, s));
+    SDTFE_HIP(hipStreamSynchronize(s));
+    const double w = bb[2] - bb[0], h = bb[3] - bb[1];
+    // about two cells per triangle, square cells, at most 4096 per side
+    const double area = w * h;
+    if (area > 0) {
+      const double cell = std::sqrt(area / (2.0 * n_tri));
+      g.nx = (int)std::min(4096.0, std::max(1.0, std::ceil(w / cell)));
+      g.ny = (int)std::min(4096.0, std::max(1.0, std::ceil(h / cell)));
+    }
+  }
+  g.x0 = bb[0]; g.y0 = bb[1]; g.x1 = bb[2]; g.y1 = bb[3];
+  g.sx = (g.x1 > g.x0) ? g.nx / (g.x1 - g.x0) : 0.0;
+  g.sy = (g.y1 > g.y0) ? g.ny / (g.y1 - g.y0) : 0.0;
+  const int n_cell = g.nx * g.ny;
+  int *ccnt = nullptr, *ccur = nullptr, *ctmp = nullptr;
+  SDTFE_RC(sc.get(&ccnt, (size_t)n_cell + 1));
+  SDTFE_RC(sc.get(&ccur, (size_t)n_cell + 1));
+  SDTFE_RC(sc.get(&ctmp, scan_tmp_ints(n_cell + 1)));
+  SDTFE_RC(dev_alloc(t, &t->cell_off, (size_t)n_cell + 1));
+  SDTFE_HIP(hipMemsetAsync(ccnt, 0, sizeof(int) * ((size_t)n_cell + 1), s));
+  SDTFE_HIP(hipMemsetAsync(ccur, 0, sizeof(int) * ((size_t)n_cell + 1), s));
+  if (n_tri > 0)
+    cell_bin_kernel<false><<<blocks_for(n_tri, 256), 256, 0, s>>>(g, ccnt, nullptr, nullptr);
+  int n_entries = 0;
+  SDTFE_RC(scan_exclusive(ccnt, t->cell_off, (int64_t)n_cell + 1, ctmp, &n_entries, s));
+  SDTFE_RC(dev_alloc(t, &t->cell_tri, (size_t)n_entries));
+  g.cell_off = t->cell_off;
+  if (n_tri > 0)
+    cell_bin_kernel<true><<<blocks_for(n_tri, 256), 256, 0, s>>>(g, ccur, t->cell_off,
+                                                                 t->cell_tri);
+  g.cell_tri = t->cell_tri;
+  t->grid = g;
+  SDTFE_HIP(hipGetLastError());
+  SDTFE_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace sdtfe

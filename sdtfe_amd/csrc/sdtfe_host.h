@@ -1,0 +1,76 @@
+// sdtfe_host.h -- host-side state and launch functions of libsdtfe_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/sdtfe_amd.h"
+#include "sdtfe_internal.h"
+
+namespace sdtfe {
+
+// Sets the calling thread's last error; returns code.
+int fail(int code, const std::string &msg);
+// Maps a HIP status to SDTFE_EHIP with context; 0 if ok.
+int hip_check(hipError_t e, const char *what);
+
+#define SDTFE_HIP(call)                                     \
+  do {                                                      \
+    int _rc = ::sdtfe::hip_check((call), #call);            \
+    if (_rc) return _rc;                                    \
+  } while (0)
+
+// Device-wide exclusive scan of n ints (n < 2^28); in and out may alias.
+// tmp needs scan_tmp_ints(n) ints.  Returns the total through *total_host
+// when non-null (synchronises the stream).
+size_t scan_tmp_ints(int64_t n);
+int scan_exclusive(const int *in, int *out, int64_t n, int *tmp, int *total_host,
+                   hipStream_t s);
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace sdtfe
+
+// Device-resident tessellation (opaque to callers).
+struct sdtfe_tess {
+  int n = 0, nt = 0, n_tri = 0;
+  float p_mass = 1.f;
+  bool given_field = false;  // vertex values taken from particle_data[:,3]
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // particles and tets as uploaded, plus the field
+  double *pos4 = nullptr;     // N x 4 (x, y, z, rho)
+  int *tets = nullptr;        // T x 8
+  double *vol = nullptr;      // T
+  double *nabla = nullptr;    // T x 3
+  sdtfe::Rec64 *rec64 = nullptr;
+  sdtfe::Rec32 *rec32 = nullptr;
+  sdtfe::Field32 *fld32 = nullptr;
+  double org[3] = {0, 0, 0};  // origin of the FP32 records
+  double span = 0;            // max |coordinate - origin|
+  // entry structure
+  double *tri_xy = nullptr;
+  int *tri_tet = nullptr;
+  int *tri_abc = nullptr;     // F x 3 vertex ids, for inspection
+  int *cell_off = nullptr;
+  int *cell_tri = nullptr;
+  sdtfe::EntryGrid grid{};
+  // per-call scratch
+  unsigned long long *stats = nullptr;  // ST_COUNT counters
+  double *map_buf = nullptr;            // G*G pixel values
+  int *navg_buf = nullptr;
+  size_t map_cap = 0;
+  int64_t device_bytes = 0;
+  double t_setup_ms = 0;
+};
+
+namespace sdtfe {
+// field.hip
+int build_field(sdtfe_tess *t, const double *particle_data, const int *tetra_data);
+int build_entry_grid(sdtfe_tess *t);
+}  // namespace sdtfe

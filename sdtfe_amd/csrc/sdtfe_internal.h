@@ -1,0 +1,116 @@
+// sdtfe_internal.h -- device data layout and kernels shared by the
+// translation units of libsdtfe_amd.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sdtfe {
+
+// ---------------------------------------------------------------- layout
+
+// Walk record of the FP64 walk, one per tet, 144 B.  Everything one step of
+// the walk reads sits in one record, so a step costs a single dependent load
+// (the record named by the previous record's neighbour slot) issued as nine
+// 16-byte loads.
+//   nb[f]  tet across the face opposite vertex f (-1 on the hull)
+//   v      A, B, C, D coordinates (x, y, z each)
+//   rho_a  density at vertex A; g the tet's density gradient
+struct alignas(16) Rec64 {
+  int nb[4];
+  double v[12];
+  double rho_a;
+  double g[3];
+};
+static_assert(sizeof(Rec64) == 144, "Rec64 layout");
+
+// Mixed-precision walk: 64 B geometry record (coordinates relative to the
+// map origin, FP32) and a 16 B field record.
+struct alignas(16) Rec32 {
+  int nb[4];
+  float v[12];
+};
+static_assert(sizeof(Rec32) == 64, "Rec32 layout");
+
+struct alignas(16) Field32 {
+  float rho_a, gx, gy, gz;
+};
+
+// Entry lookup: forward-hull triangles binned on a uniform 2D grid.
+struct EntryGrid {
+  const double *tri_xy;   // F x 6: a.x a.y b.x b.y c.x c.y
+  const int *tri_tet;     // F
+  const int *cell_off;    // nx*ny + 1
+  const int *cell_tri;    // triangle ids per cell
+  double x0, y0, x1, y1;  // grid extent (bounding box of the triangles)
+  double sx, sy;          // cells per unit length
+  int nx, ny, n_tri;
+};
+
+// Everything the walk needs to place samples (src/dtfe.c:579-589, 613-622).
+struct MapGeom {
+  double dxy, off_x, off_y, off_z, lz, scale, soff;
+  double org_x, org_y, org_z;  // origin of the FP32 coordinates
+  uint64_t seed;
+  int grid, samples, max_steps;
+};
+
+enum StatIdx {
+  ST_SAMPLES = 0,
+  ST_MISSED,
+  ST_FAILED,
+  ST_NO_ISECT,
+  ST_PERT_EXHAUSTED,
+  ST_NO_ISECT_PERT,
+  ST_PERT_STEPS,
+  ST_TET_STEPS,
+  ST_STEP_CAP,
+  ST_FP64_REDO,
+  ST_COUNT
+};
+
+// ---------------------------------------------------------------- stream
+
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  // SplitMix64 output function (Steele, Lea, Flood, OOPSLA 2014).
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+
+// Key of one (seed, pixel, sample); draws are draw31(key, d).
+__host__ __device__ inline uint64_t sample_key(uint64_t seed, uint64_t pixel, uint32_t sample) {
+  uint64_t k = mix64(seed + kGamma);
+  k = mix64(k ^ pixel);
+  return mix64(k ^ (uint64_t)sample);
+}
+
+__host__ __device__ inline uint32_t draw31(uint64_t key, uint32_t draw) {
+  return (uint32_t)(mix64(key + (uint64_t)(draw + 1u) * kGamma) >> 33);
+}
+
+// --------------------------------------------------------- wave helpers
+
+__device__ inline void add_stat(unsigned long long *stats, int idx, unsigned long long v) {
+  // one atomic per wave: sum over the wave's lanes first
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(stats + idx, v);
+}
+
+// --------------------------------------------------------- launch helpers
+
+// Blocks are dealt to the 8 XCDs round-robin; remap so that consecutive
+// logical tiles (neighbouring pixels, which walk the same tets) land on the
+// same XCD and share its L2.
+__device__ inline int xcd_tile(int block, int n_blocks) {
+  const int n_xcd = 8;
+  int per = n_blocks / n_xcd;
+  int full = per * n_xcd;
+  if (block >= full) return block;
+  return (block % n_xcd) * per + block / n_xcd;
+}
+
+}  // namespace sdtfe

@@ -1,0 +1,90 @@
+// walk_common.h -- device code shared by the walk kernels: entry lookup,
+// sample placement and per-lane counters.
+#pragma once
+
+#include <climits>
+
+#include "sdtfe_internal.h"
+
+namespace sdtfe {
+
+__device__ inline int grid_cell(double v, double v0, double scale, int n) {
+  int c = (int)floor((v - v0) * scale);
+  return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+// Point-in-triangle with the reference's inclusive convention
+// (src/dtfe.c:528-545): outside only if the query and the opposite vertex
+// lie strictly on opposite sides of an edge.  Same expressions, same order.
+__device__ inline bool tri_contains(const double *xy, double qx, double qy) {
+  const double ax = xy[0], ay = xy[1], bx = xy[2], by = xy[3], cx = xy[4], cy = xy[5];
+  double d_ins = (cx - ax) * (by - ay) - (cy - ay) * (bx - ax);
+  double d_qry = (qx - ax) * (by - ay) - (qy - ay) * (bx - ax);
+  if (d_ins * d_qry < 0.0) return false;
+  d_ins = (ax - bx) * (cy - by) - (ay - by) * (cx - bx);
+  d_qry = (qx - bx) * (cy - by) - (qy - by) * (cx - bx);
+  if (d_ins * d_qry < 0.0) return false;
+  d_ins = (bx - cx) * (ay - cy) - (by - cy) * (ax - cx);
+  d_qry = (qx - cx) * (ay - cy) - (qy - cy) * (ax - cx);
+  return !(d_ins * d_qry < 0.0);
+}
+
+// Entry tet of the vertical ray through (x, y): the forward-hull triangle
+// containing the point (lowest triangle number if it lies on a shared edge),
+// -1 outside the hull's shadow.  Replaces the visibility walk of pt_loc_2d
+// (src/dtfe.c:518-552); both return the same tet except for points on a
+// shared edge, where either neighbour is a valid entry.
+__device__ inline int locate_entry(const EntryGrid &g, double x, double y) {
+  if (g.n_tri == 0) return -1;
+  if (!(x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1)) return -1;
+  const int c = grid_cell(y, g.y0, g.sy, g.ny) * g.nx + grid_cell(x, g.x0, g.sx, g.nx);
+  const int b = g.cell_off[c], e = g.cell_off[c + 1];
+  int best = INT_MAX;
+  for (int i = b; i < e; ++i) {
+    const int k = g.cell_tri[i];
+    if (k < best && tri_contains(g.tri_xy + 6 * (int64_t)k, x, y)) best = k;
+  }
+  return best == INT_MAX ? -1 : g.tri_tet[best];
+}
+
+// Lower-left corner of pixel (px, py)'s sampling square and one sample in
+// it (src/dtfe.c:613-622: x drawn before y, r / RAND_MAX in [0, 1]).
+__device__ inline void sample_point(const MapGeom &g, int px, int py, uint64_t key, double &xt,
+                                    double &yt) {
+  const double x = px * g.dxy + g.off_x - g.soff;
+  const double y = py * g.dxy + g.off_y - g.soff;
+  xt = x + ((double)draw31(key, 0) / 2147483647.0) * g.scale;
+  yt = y + ((double)draw31(key, 1) / 2147483647.0) * g.scale;
+}
+
+struct LaneCounters {
+  unsigned missed = 0, failed = 0, no_isect = 0, pert_exh = 0, no_isect_pert = 0;
+  unsigned pert_steps = 0, step_cap = 0, fp64_redo = 0;
+  unsigned long long samples = 0, tet_steps = 0;
+
+  __device__ void flush(unsigned long long *st) {
+    add_stat(st, ST_SAMPLES, samples);
+    add_stat(st, ST_MISSED, missed);
+    add_stat(st, ST_FAILED, failed);
+    add_stat(st, ST_NO_ISECT, no_isect);
+    add_stat(st, ST_PERT_EXHAUSTED, pert_exh);
+    add_stat(st, ST_NO_ISECT_PERT, no_isect_pert);
+    add_stat(st, ST_PERT_STEPS, pert_steps);
+    add_stat(st, ST_TET_STEPS, tet_steps);
+    add_stat(st, ST_STEP_CAP, step_cap);
+    add_stat(st, ST_FP64_REDO, fp64_redo);
+  }
+};
+
+// Pixel of a lane: blocks of 256 lanes cover 16x16 pixels as four 8x8
+// wave tiles; block ids are remapped so neighbouring tiles share an XCD.
+__device__ inline void lane_pixel(int grid, int &px, int &py) {
+  const int nbx = (grid + 15) >> 4;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nbx, by = tile / nbx;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  px = bx * 16 + (w & 1) * 8 + (lane & 7);
+  py = by * 16 + (w >> 1) * 8 + (lane >> 3);
+}
+
+}  // namespace sdtfe

@@ -1,0 +1,165 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's golden
+vectors and the CPU oracle on the same inputs and the same sample stream.
+
+Bar: integer and index results (entry tets, tets crossed, triangles, n_avg)
+exactly equal; FP64 results within 1e-10 relative per value (the FP64 path
+keeps the reference's operation order, so in practice they are bit-equal,
+which is also reported).
+"""
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+from sdtfe_amd import capi, workloads
+from sdtfe_amd.tessellate import to_particle_data, triangulate
+
+pytestmark = pytest.mark.gpu
+
+RTOL_FP64 = 1e-10
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    d = np.abs(a - b)
+    return np.max(d / np.maximum(np.abs(b), 1e-300)) if a.size else 0.0
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    n = capi.device_count()
+    assert n >= 1, "no HIP device visible to libsdtfe_amd"
+
+
+@pytest.mark.parametrize("case", ["tet4", "pts8", "pts64", "dyadic"])
+def test_field_matches_reference(golden, case):
+    _say(f"[gpu] field {case}")
+    g = golden("g1_tiny.npz")
+    pd = to_particle_data(g[f"{case}_pos"])
+    with capi.Tessellation(pd, g[f"{case}_tetra"], p_mass=0.75) as t:
+        np.testing.assert_array_equal(t.vertex_density(), g[f"{case}_rho"])
+        np.testing.assert_array_equal(t.gradients(), g[f"{case}_nabla"])
+        tri = t.triangles()
+        ref = g[f"{case}_tri"]
+        np.testing.assert_array_equal(tri[:, [0, 1, 2, 6]], ref[:, [0, 1, 2, 6]])
+
+
+def test_entry_and_rays_match_reference(golden):
+    _say("[gpu] rays g2 (fp64)")
+    g = golden("g2_rays.npz")
+    pd = to_particle_data(g["pos"])
+    with capi.Tessellation(pd, g["tetra"]) as t:
+        np.testing.assert_array_equal(t.vertex_density(), g["rho"])
+        entry = t.locate(g["x"], g["y"])
+        assert (entry == g["entry"]).all(), np.nonzero(entry != g["entry"])[0][:10]
+        val, crossed, st = t.trace(g["x"], g["y"], g["pixel"], g["sample"], float(g["off_z"]),
+                                   float(g["lz"]), float(g["dxy"]), int(g["seed"]),
+                                   entry=g["entry"])
+        np.testing.assert_array_equal(crossed, g["crossed"])
+        assert _rel(val, g["value"]) <= RTOL_FP64
+        n_exact = int((val == g["value"]).sum())
+        _say(f"  rays bit-equal {n_exact}/{val.size}, perturb steps {st['perturb_steps']}")
+        assert st["perturb_steps"] > 100  # crafted rays exercised the perturbation
+
+
+MAPS = __import__("tests.golden.make_golden", fromlist=["MAPS"]).MAPS
+
+
+@pytest.mark.parametrize("name,G,S,L,Lz,cen,ds,m,seed", [pytest.param(*m, id=m[0]) for m in MAPS])
+def test_maps_match_reference(golden, name, G, S, L, Lz, cen, ds, m, seed):
+    _say(f"[gpu] map {name}")
+    g = golden("g3_maps.npz")
+    pd = to_particle_data(g["pos"])
+    with capi.Tessellation(pd, g["tetra"], p_mass=m) as t:
+        np.testing.assert_array_equal(t.vertex_density(), g[f"{name}_vertex_rho"])
+        rho, st = t.map(G, S, box_len=L, box_len_z=Lz, center=cen, delta_sample=ds, seed=seed)
+    assert _rel(rho, g[f"{name}_rho"]) <= RTOL_FP64
+    _say(f"  bit-equal pixels {(rho == g[f'{name}_rho']).sum()}/{rho.size}")
+    assert st["samples"] == G * G * S
+
+
+def test_uniform_32768_matches_reference(golden):
+    _say("[gpu] uniform-32768 256x256x4 vs reference golden")
+    g = golden("g4_c1.npz")
+    pos = workloads.uniform(32768, 0)
+    tetra = triangulate(pos)
+    if hashlib.sha256(tetra.tobytes()).hexdigest() != str(g["tetra_sha"]):
+        pytest.skip("scipy/Qhull tessellates differently from the fixture")
+    spec = workloads.C1["spec"]
+    with capi.Tessellation(to_particle_data(pos), tetra, p_mass=spec.p_mass) as t:
+        rho, st, navg = t.map(256, 4, want_navg=True, box_len=spec.box_len,
+                              box_len_z=spec.box_len_z, center=spec.center,
+                              delta_sample=spec.delta_sample, seed=int(g["seed"]))
+    assert _rel(rho, g["rho"]) <= RTOL_FP64
+    _say(f"  bit-equal pixels {(rho == g['rho']).sum()}/{rho.size}; steps {st['tet_steps']}")
+    assert st["samples"] == 256 * 256 * 4 and st["failed"] == 0
+
+
+def test_clustered_map_matches_oracle(oracle):
+    """A clustered set (halos with steep density) at a size the oracle
+    finishes in seconds; per-pixel values and n_avg against the oracle."""
+    _say("[gpu] clustered-65536 256x256x8 vs oracle")
+    pos = workloads.clustered(65536, seed=3)
+    tetra = triangulate(pos)
+    pd = to_particle_data(pos)
+    G, S, seed = 256, 8, 21
+    ref, st_o, navg_o = oracle.compute_density(pd.copy(), tetra, G, 1.0, 1.0, 1.0,
+                                               (0.5, 0.5, 0.5), S, 0.0, seed, want_navg=True)
+    with capi.Tessellation(pd, tetra) as t:
+        rho, st, navg = t.map(G, S, want_navg=True, seed=seed)
+    np.testing.assert_array_equal(navg, navg_o)
+    assert _rel(rho, ref) <= RTOL_FP64
+    assert st["tet_steps"] == st_o["tet_steps"]
+    assert st["perturb_steps"] == st_o["perturb_steps"]
+
+
+def test_dropin_compute_density(golden):
+    """The reference entry point through the ABI: rho accumulates, column 3
+    receives the vertex density, and the seed advances per call."""
+    _say("[gpu] drop-in compute_density")
+    g = golden("g3_maps.npz")
+    name, G, S, L, Lz, cen, ds, m, seed = MAPS[0]
+    pd = to_particle_data(g["pos"])
+    rho = np.full(G * G, 0.5)
+    capi.set_seed(seed)
+    capi.compute_density(pd, g["tetra"], G, L, Lz, rho, m, cen, S, ds)
+    np.testing.assert_array_equal(pd[:, 3], g[f"{name}_vertex_rho"])
+    assert _rel(rho - 0.5, g[f"{name}_rho"]) <= 1e-9
+    assert capi.lib().sdtfe_get_seed() == seed + 1
+
+
+def test_map_device_accumulates_and_repeats():
+    _say("[gpu] map_device accumulate + repeatability")
+    pos = workloads.uniform(20000, 4)
+    tetra = triangulate(pos)
+    with capi.Tessellation(to_particle_data(pos), tetra) as t:
+        host, _ = t.map(128, 4, seed=9)
+        buf = capi.DeviceBuffer(128 * 128 * 8)
+        buf.zero()
+        p = t.params(128, 4, seed=9)
+        t.map_device(p, buf)
+        t.map_device(p, buf)
+        dev = buf.to_host()
+        buf.close()
+    np.testing.assert_array_equal(dev, host + host)
+
+
+def test_errors_are_reported():
+    _say("[gpu] error paths")
+    pos = workloads.uniform(200, 1)
+    tetra = triangulate(pos)
+    bad = tetra.copy()
+    bad[3, 5] = tetra.shape[0] + 7
+    with pytest.raises(capi.SdtfeError, match="out of range"):
+        capi.Tessellation(to_particle_data(pos), bad)
+    with capi.Tessellation(to_particle_data(pos), tetra) as t:
+        with pytest.raises(capi.SdtfeError):
+            t.map(0, 4)
+        rho, st = t.map(16, 2, box_len=3.0)  # most samples outside the hull's shadow
+        assert st["missed"] > 0 and np.isfinite(rho).all()
