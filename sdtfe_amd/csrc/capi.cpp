@@ -1,5 +1,6 @@
 // capi.cpp -- extern "C" entry points of libsdtfe_amd.so (declared in
 // include/sdtfe_amd.h).  Host C++ over the HIP runtime; no torch types.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -63,6 +64,11 @@ MapGeom make_geom(const sdtfe_tess *t, const sdtfe_map_params &p) {
   g.grid = p.grid_dim;
   g.samples = p.n_mc_samp;
   g.max_steps = p.max_steps > 0 ? p.max_steps : SDTFE_MAX_STEPS_DEFAULT;
+  // samples fall in [off - soff, off - soff + (G - 1) dxy + scale]
+  const double lo_x = g.off_x - g.soff, hi_x = lo_x + (p.grid_dim - 1) * g.dxy + g.scale;
+  const double lo_y = g.off_y - g.soff, hi_y = lo_y + (p.grid_dim - 1) * g.dxy + g.scale;
+  g.ray_span = std::max(std::max(std::fabs(lo_x - g.org_x), std::fabs(hi_x - g.org_x)),
+                        std::max(std::fabs(lo_y - g.org_y), std::fabs(hi_y - g.org_y)));
   return g;
 }
 
@@ -304,6 +310,9 @@ int sdtfe_tess_trace(sdtfe_tess *t, int precision, int n, const double *x, const
   g.org_z = t->org[2];
   g.seed = seed;
   g.max_steps = SDTFE_MAX_STEPS_DEFAULT;
+  g.ray_span = 0;
+  for (int i = 0; i < n; ++i)
+    g.ray_span = std::max(g.ray_span, std::max(std::fabs(x[i] - g.org_x), std::fabs(y[i] - g.org_y)));
   SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
   if (n > 0) {
     DevArray<double> dx, dy, dv;

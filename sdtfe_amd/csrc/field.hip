@@ -247,7 +247,7 @@ __global__ void gradient_record_kernel(const double4 *pos, const int *tets, int 
     q.v[3 * k + 2] = (float)(p4[k]->z - oz);
   }
   rec32[t] = q;
-  fld[t] = Field32{(float)a.w, (float)gx, (float)gy, (float)gz};
+  fld[t] = Field32{(float)a.w, (float)b.w, (float)c.w, (float)d.w};
 }
 
 // ============================================================ entry grid

@@ -6,6 +6,10 @@
 
 #include <cstdint>
 
+// Pure arithmetic helpers are host-callable too, so the same walk code can
+// be exercised on the CPU.
+#define SDTFE_HD __host__ __device__
+
 namespace sdtfe {
 
 // ---------------------------------------------------------------- layout
@@ -26,7 +30,8 @@ struct alignas(16) Rec64 {
 static_assert(sizeof(Rec64) == 144, "Rec64 layout");
 
 // Mixed-precision walk: 64 B geometry record (coordinates relative to the
-// map origin, FP32) and a 16 B field record.
+// tessellation's origin, FP32) and a 16 B record of the four vertex
+// densities (FP32).
 struct alignas(16) Rec32 {
   int nb[4];
   float v[12];
@@ -34,7 +39,7 @@ struct alignas(16) Rec32 {
 static_assert(sizeof(Rec32) == 64, "Rec32 layout");
 
 struct alignas(16) Field32 {
-  float rho_a, gx, gy, gz;
+  float rho_a, rho_b, rho_c, rho_d;
 };
 
 // Entry lookup: forward-hull triangles binned on a uniform 2D grid.
@@ -52,6 +57,7 @@ struct EntryGrid {
 struct MapGeom {
   double dxy, off_x, off_y, off_z, lz, scale, soff;
   double org_x, org_y, org_z;  // origin of the FP32 coordinates
+  double ray_span;             // max |ray x or y - origin| over the launch
   uint64_t seed;
   int grid, samples, max_steps;
 };

@@ -8,7 +8,7 @@
 
 namespace sdtfe {
 
-__device__ inline int grid_cell(double v, double v0, double scale, int n) {
+SDTFE_HD inline int grid_cell(double v, double v0, double scale, int n) {
   int c = (int)floor((v - v0) * scale);
   return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
@@ -16,7 +16,7 @@ __device__ inline int grid_cell(double v, double v0, double scale, int n) {
 // Point-in-triangle with the reference's inclusive convention
 // (src/dtfe.c:528-545): outside only if the query and the opposite vertex
 // lie strictly on opposite sides of an edge.  Same expressions, same order.
-__device__ inline bool tri_contains(const double *xy, double qx, double qy) {
+SDTFE_HD inline bool tri_contains(const double *xy, double qx, double qy) {
   const double ax = xy[0], ay = xy[1], bx = xy[2], by = xy[3], cx = xy[4], cy = xy[5];
   double d_ins = (cx - ax) * (by - ay) - (cy - ay) * (bx - ax);
   double d_qry = (qx - ax) * (by - ay) - (qy - ay) * (bx - ax);
@@ -34,7 +34,7 @@ __device__ inline bool tri_contains(const double *xy, double qx, double qy) {
 // -1 outside the hull's shadow.  Replaces the visibility walk of pt_loc_2d
 // (src/dtfe.c:518-552); both return the same tet except for points on a
 // shared edge, where either neighbour is a valid entry.
-__device__ inline int locate_entry(const EntryGrid &g, double x, double y) {
+SDTFE_HD inline int locate_entry(const EntryGrid &g, double x, double y) {
   if (g.n_tri == 0) return -1;
   if (!(x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1)) return -1;
   const int c = grid_cell(y, g.y0, g.sy, g.ny) * g.nx + grid_cell(x, g.x0, g.sx, g.nx);
@@ -49,7 +49,7 @@ __device__ inline int locate_entry(const EntryGrid &g, double x, double y) {
 
 // Lower-left corner of pixel (px, py)'s sampling square and one sample in
 // it (src/dtfe.c:613-622: x drawn before y, r / RAND_MAX in [0, 1]).
-__device__ inline void sample_point(const MapGeom &g, int px, int py, uint64_t key, double &xt,
+SDTFE_HD inline void sample_point(const MapGeom &g, int px, int py, uint64_t key, double &xt,
                                     double &yt) {
   const double x = px * g.dxy + g.off_x - g.soff;
   const double y = py * g.dxy + g.off_y - g.soff;

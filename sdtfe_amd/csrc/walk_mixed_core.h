@@ -1,0 +1,364 @@
+// walk_mixed_core.h -- the mixed-precision ray walk (gfx950).
+//
+// Same tets, same faces, same perturbations as the FP64 walk; faster
+// arithmetic and a smaller record:
+//
+//  * Side tests in FP32 with a filter.  The Plücker product of the ray with
+//    edge (X, Y) is lz * orient2d(p; X, Y) (src/dtfe.c:24-26 expanded), so it
+//    is formed in FP32 as (X - p) x (Y - p) from coordinates stored relative
+//    to the tessellation's origin.  An a-priori bound covers the rounding of
+//    the stored coordinates, of the differences and of the products; the
+//    FP32 sign is taken only when |c| exceeds that bound plus the reference's
+//    zero band (ZERO = 1e4 DBL_EPSILON, src/constants.h:6, scaled by 1/lz)
+//    plus the FP64 expression's own error.  Otherwise the product is redone
+//    in FP64 with the reference expression and Sign().  Every side sign is
+//    therefore the FP64 walk's sign, so the walk visits the same tets.
+//  * A tet whose face choice is special (a zero side) or fails is handed to
+//    the FP64 core, which perturbs the ray exactly as the FP64 walk does.
+//  * The crossing height of each face is computed once (leave face, FP32) and
+//    carried as the next tet's entry height; errors in a crossing height then
+//    cancel to first order because the DTFE field is continuous across the
+//    face.  Midpoints near the slab boundary are decided in FP64.
+//  * The density at each crossing is the barycentric mix of the face's
+//    vertex densities (same weights as the height) and is carried with it;
+//    the field being linear along the chord, the trapezoid rule on the two
+//    crossings is its exact integral.  This never forms rho_A + grad.(p - A),
+//    which is ill-conditioned in FP32 for sliver tets.  Per-ray and per-pixel
+//    sums stay FP64; tet volumes and vertex densities are the FP64 ones.
+//
+// Per step: one 64 B geometry record + one 16 B record of vertex densities.
+#pragma once
+
+#include <algorithm>
+
+#include "sdtfe_host.h"
+#include "walk_fp64_core.h"
+
+namespace sdtfe {
+
+// Largest tolerated relative uncertainty of FP32 barycentric weights.
+constexpr float kCondMax = 1e-4f;
+
+struct Tet32 {
+  int nb0, nb1, nb2, nb3;
+  float ax, ay, az, bx, by, bz, cx, cy, cz, dx, dy, dz;
+};
+
+SDTFE_HD inline void load_tet32(const Rec32 *__restrict__ rec, int t, Tet32 &r) {
+  const float4 *p = reinterpret_cast<const float4 *>(rec + t);
+  const int4 nb = *reinterpret_cast<const int4 *>(p);
+  const float4 q0 = p[1], q1 = p[2], q2 = p[3];
+  r.nb0 = nb.x; r.nb1 = nb.y; r.nb2 = nb.z; r.nb3 = nb.w;
+  r.ax = q0.x; r.ay = q0.y; r.az = q0.z;
+  r.bx = q0.w; r.by = q1.x; r.bz = q1.y;
+  r.cx = q1.z; r.cy = q1.w; r.cz = q2.x;
+  r.dx = q2.y; r.dy = q2.z; r.dz = q2.w;
+}
+
+SDTFE_HD inline int nb32(const Tet32 &r, int f) {
+  return f == 0 ? r.nb0 : (f == 1 ? r.nb1 : (f == 2 ? r.nb2 : r.nb3));
+}
+
+// Per-map constants of the filter (host-computed, see filter_consts()).
+struct Filter {
+  float e;       // bound on |(X - p) stored - (X - p) exact| per component
+  float k_prod;  // relative rounding of the products and difference
+  float thr;     // (ZERO + FP64 expression error) / lz
+};
+
+// Sign of one side product, and its FP32 value for the barycentric weights.
+// (x1, y1) = X - p, (x2, y2) = Y - p in FP32; l1 = |x|+|y| of each.
+struct Side {
+  float c;
+  float b;  // bound on |c - exact|
+  int s;
+};
+
+SDTFE_HD inline Side side32(float x1, float y1, float l1a, float x2, float y2, float l1b,
+                              const Filter &f, bool &need64) {
+  const float p1 = x1 * y2, p2 = y1 * x2;
+  const float c = p1 - p2;
+  const float bound = f.k_prod * (fabsf(p1) + fabsf(p2)) + f.e * (l1a + l1b) + f.thr;
+  Side s;
+  s.c = c;
+  s.b = bound;
+  s.s = c > bound ? 1 : (c < -bound ? -1 : 0);
+  need64 = s.s == 0;
+  return s;
+}
+
+// Crossing point of face f from FP32 side values: height and density there,
+// both the barycentric mix of the face's three vertices with the weights the
+// FP64 walk uses for the height (face_z), written relative to the face's
+// first vertex.  rho holds the vertex densities (A, B, C, D).
+struct Cross {
+  float z, rho;
+};
+
+SDTFE_HD inline Cross face_cross32(int f, float cCB, float cAC, float cBA, float cDC, float cBD,
+                                   float cDA, const Tet32 &r, const float4 &rho) {
+  float t0, t1, t2, w1n, w2n, z0, z1, z2, q0, q1, q2;
+  if (f == 0) {  // BDC: D + w1 (C - D) + w2 (B - D)
+    t0 = -cCB; t1 = cDC; t2 = -cBD; w1n = t2; w2n = t1;
+    z0 = r.dz; z1 = r.cz; z2 = r.bz; q0 = rho.w; q1 = rho.z; q2 = rho.y;
+  } else if (f == 1) {  // ACD: C + w1 (D - C) + w2 (A - C)
+    t0 = cAC; t1 = -cDC; t2 = -cDA; w1n = t0; w2n = t1;
+    z0 = r.cz; z1 = r.dz; z2 = r.az; q0 = rho.z; q1 = rho.w; q2 = rho.x;
+  } else if (f == 2) {  // ADB: B + w1 (A - B) + w2 (D - B)
+    t0 = cDA; t1 = cBD; t2 = -cBA; w1n = t1; w2n = t2;
+    z0 = r.bz; z1 = r.az; z2 = r.dz; q0 = rho.y; q1 = rho.x; q2 = rho.w;
+  } else {  // ABC: A + w1 (B - A) + w2 (C - A)
+    t0 = cBA; t1 = cCB; t2 = -cAC; w1n = t2; w2n = t0;
+    z0 = r.az; z1 = r.bz; z2 = r.cz; q0 = rho.x; q1 = rho.y; q2 = rho.z;
+  }
+  const float inv = 1.0f / (t0 + t1 + t2);
+  const float w1 = w1n * inv, w2 = w2n * inv;
+  Cross c;
+  c.z = fmaf(w1, z1 - z0, fmaf(w2, z2 - z0, z0));
+  c.rho = fmaf(w1, q1 - q0, fmaf(w2, q2 - q0, q0));
+  return c;
+}
+
+// Same crossing from the FP64 record: side products with the reference
+// expression, weights and height in FP64 (the density mixes the FP32 vertex
+// densities).  Height returned relative to org_z.
+SDTFE_HD inline Cross face_cross64(int f, const Tet64 &q, double lz, double ox, double oy,
+                                   const float4 &rho, double org_z) {
+  const double p2 = lz, p3 = -lz * oy, p4 = lz * ox;
+  const double uCB = edge_u(p2, p3, p4, q.cx, q.cy, q.bx, q.by);
+  const double uAC = edge_u(p2, p3, p4, q.cx, q.cy, q.ax, q.ay);
+  const double uBA = edge_u(p2, p3, p4, q.bx, q.by, q.ax, q.ay);
+  const double uDC = edge_u(p2, p3, p4, q.cx, q.cy, q.dx, q.dy);
+  const double uBD = edge_u(p2, p3, p4, q.bx, q.by, q.dx, q.dy);
+  const double uDA = edge_u(p2, p3, p4, q.dx, q.dy, q.ax, q.ay);
+  double t0, t1, t2, w1n, w2n, z0, z1, z2, r0, r1, r2;
+  if (f == 0) {
+    t0 = -uCB; t1 = uDC; t2 = -uBD; w1n = t2; w2n = t1;
+    z0 = q.dz; z1 = q.cz; z2 = q.bz; r0 = rho.w; r1 = rho.z; r2 = rho.y;
+  } else if (f == 1) {
+    t0 = uAC; t1 = -uDC; t2 = -uDA; w1n = t0; w2n = t1;
+    z0 = q.cz; z1 = q.dz; z2 = q.az; r0 = rho.z; r1 = rho.w; r2 = rho.x;
+  } else if (f == 2) {
+    t0 = uDA; t1 = uBD; t2 = -uBA; w1n = t1; w2n = t2;
+    z0 = q.bz; z1 = q.az; z2 = q.dz; r0 = rho.y; r1 = rho.x; r2 = rho.w;
+  } else {
+    t0 = uBA; t1 = uCB; t2 = -uAC; w1n = t2; w2n = t0;
+    z0 = q.az; z1 = q.bz; z2 = q.cz; r0 = rho.x; r1 = rho.y; r2 = rho.z;
+  }
+  const double inv = 1.0 / (t0 + t1 + t2);
+  const double w1 = w1n * inv, w2 = w2n * inv;
+  Cross c;
+  c.z = (float)(z0 + w1 * (z1 - z0) + w2 * (z2 - z0) - org_z);
+  c.rho = (float)(r0 + w1 * (r1 - r0) + w2 * (r2 - r0));
+  return c;
+}
+
+// Relative uncertainty of face f's FP32 barycentric weights.
+SDTFE_HD inline float face_cond(int f, const Side &CB, const Side &AC, const Side &BA,
+                                const Side &DC, const Side &BD, const Side &DA) {
+  float den, b;
+  if (f == 0) {
+    den = -CB.c + DC.c - BD.c; b = CB.b + DC.b + BD.b;
+  } else if (f == 1) {
+    den = AC.c - DC.c - DA.c; b = AC.b + DC.b + DA.b;
+  } else if (f == 2) {
+    den = DA.c + BD.c - BA.c; b = DA.b + BD.b + BA.b;
+  } else {
+    den = BA.c + CB.c - AC.c; b = BA.b + CB.b + AC.b;
+  }
+  return b / fabsf(den);
+}
+
+// Face choice from the six side signs (src/dtfe.c:64-213, as plucker64).
+SDTFE_HD inline void choose_faces(int sCB, int sAC, int sBA, int sDC, int sBD, int sDA,
+                                    int &enter, int &leave, bool &ok, bool &special) {
+  const int fa[4] = {-sCB, sAC, sDA, sBA};
+  const int fb[4] = {sDC, -sDC, sBD, sCB};
+  const int fc[4] = {-sBD, -sDA, -sBA, -sAC};
+  enter = -1;
+  leave = -1;
+  special = false;
+  int found = 0;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const int a = fa[f], b = fb[f], c = fc[f];
+    const bool active = found < 2 && (a | b | c) != 0;
+    const bool en = active && enter == -1 && a >= 0 && b >= 0 && c >= 0;
+    const bool le = active && !en && leave == -1 && a <= 0 && b <= 0 && c <= 0;
+    if (en) enter = f;
+    if (le) leave = f;
+    if ((en || le) && (a == 0 || b == 0 || c == 0)) special = true;
+    found += (en || le) ? 1 : 0;
+  }
+  ok = found == 2;
+}
+
+// One sample-ray.  (x, y) FP64 sample position; the walk keeps the FP64
+// position (changed only by perturbations) and its FP32 offset from the
+// origin.
+SDTFE_HD inline double walk_mixed(const Rec32 *__restrict__ rec32, const Field32 *__restrict__ fld,
+                             const Rec64 *__restrict__ rec64, int entry, double x, double y,
+                             const MapGeom &g, const Filter &flt, float zlo, float zhi,
+                             float zband, uint64_t key, int &crossed, LaneCounters &cnt) {
+  crossed = 0;
+  if (entry < 0) return 0.0;
+  const double step = (g.dxy / 2.0) / kPerturbMax;
+  double ox = x, oy = y, acc = 0.0;
+  float rx = (float)(x - g.org_x), ry = (float)(y - g.org_y);
+  uint32_t draw = 2;
+  int cur = entry, prev = -1, f_in = -1, f_out = 0, nxt = entry, steps = 0;
+  Cross carry{0.f, 0.f};
+  bool have_carry = false;
+  Tet32 r;
+  for (;;) {
+    if (steps > 0) nxt = nb32(r, f_out);
+    if (nxt == -1) break;
+    if (nxt == prev) {  // turn around (src/dtfe.c:303-309)
+      const int tmp = f_in;
+      f_in = f_out;
+      f_out = tmp;
+      nxt = nb32(r, f_out);
+      if (nxt == -1) break;
+      have_carry = false;
+    }
+    if (++steps > g.max_steps) {
+      cnt.step_cap++;
+      crossed = 0;
+      return 0.0;
+    }
+    prev = cur;
+    cur = nxt;
+    load_tet32(rec32, cur, r);
+    const float4 rho4 = *reinterpret_cast<const float4 *>(fld + cur);
+
+    // vertex offsets from the ray, FP32
+    const float ax = r.ax - rx, ay = r.ay - ry, bx = r.bx - rx, by = r.by - ry;
+    const float cx = r.cx - rx, cy = r.cy - ry, dx = r.dx - rx, dy = r.dy - ry;
+    const float la = fabsf(ax) + fabsf(ay), lb = fabsf(bx) + fabsf(by);
+    const float lc = fabsf(cx) + fabsf(cy), ld = fabsf(dx) + fabsf(dy);
+    bool n0, n1, n2, n3, n4, n5;
+    Side CB = side32(cx, cy, lc, bx, by, lb, flt, n0);
+    Side AC = side32(cx, cy, lc, ax, ay, la, flt, n1);
+    Side BA = side32(bx, by, lb, ax, ay, la, flt, n2);
+    Side DC = side32(cx, cy, lc, dx, dy, ld, flt, n3);
+    Side BD = side32(bx, by, lb, dx, dy, ld, flt, n4);
+    Side DA = side32(dx, dy, ld, ax, ay, la, flt, n5);
+    if (n0 | n1 | n2 | n3 | n4 | n5) {
+      // undecided sides: the reference expression in FP64
+      Tet64 q;
+      load_tet(rec64, cur, q);
+      const double p2 = g.lz, p3 = -g.lz * oy, p4 = g.lz * ox;
+      const float il = (float)(1.0 / g.lz);
+      unsigned redo = 0;
+#define SDTFE_REDO(S, N, XX, XY, YX, YY)                                 \
+  if (N) {                                                               \
+    const double u = edge_u(p2, p3, p4, q.XX, q.XY, q.YX, q.YY);         \
+    S.s = sign_zero(u);                                                  \
+    S.c = (float)u * il;                                                 \
+    ++redo;                                                              \
+  }
+      SDTFE_REDO(CB, n0, cx, cy, bx, by)
+      SDTFE_REDO(AC, n1, cx, cy, ax, ay)
+      SDTFE_REDO(BA, n2, bx, by, ax, ay)
+      SDTFE_REDO(DC, n3, cx, cy, dx, dy)
+      SDTFE_REDO(BD, n4, bx, by, dx, dy)
+      SDTFE_REDO(DA, n5, dx, dy, ax, ay)
+#undef SDTFE_REDO
+      cnt.fp64_redo += redo;
+    }
+    int enter, leave;
+    bool ok, special;
+    choose_faces(CB.s, AC.s, BA.s, DC.s, BD.s, DA.s, enter, leave, ok, special);
+    if (!ok || special) {
+      // the FP64 walk's own step, perturbation included
+      Tet64 q;
+      load_tet(rec64, cur, q);
+      Hit h = plucker64(q, g.lz, ox, oy);
+      if (!h.ok) {
+        cnt.no_isect++;
+        crossed = 0;
+        return 0.0;
+      }
+      if (h.special && !resolve_special(q, g.lz, step, key, draw, ox, oy, h, cnt)) {
+        crossed = 0;
+        return 0.0;
+      }
+      rx = (float)(ox - g.org_x);
+      ry = (float)(oy - g.org_y);
+      f_in = h.enter;
+      f_out = h.leave;
+      ++crossed;
+      acc += tet_integral64(q, ox, oy, h.z_in, h.z_out, g.off_z, g.lz);
+      have_carry = false;  // the next tet starts from its own entry face
+      continue;
+    }
+    f_in = enter;
+    f_out = leave;
+    ++crossed;
+    // A crossing is taken in FP64 when its error would not cancel against a
+    // neighbour's (the ray's first and last crossing) or when the face's FP32
+    // weights are ill-conditioned (nearly vertical faces).
+    const bool in64 = !have_carry;
+    const bool out64 = nb32(r, leave) == -1 ||
+                       face_cond(leave, CB, AC, BA, DC, BD, DA) > kCondMax;
+    Cross in = carry, out;
+    if (in64 | out64) {
+      Tet64 q;
+      load_tet(rec64, cur, q);
+      if (in64) in = face_cross64(enter, q, g.lz, ox, oy, rho4, g.org_z);
+      out = out64 ? face_cross64(leave, q, g.lz, ox, oy, rho4, g.org_z)
+                  : face_cross32(leave, CB.c, AC.c, BA.c, DC.c, BD.c, DA.c, r, rho4);
+      cnt.fp64_redo++;
+    } else {
+      out = face_cross32(leave, CB.c, AC.c, BA.c, DC.c, BD.c, DA.c, r, rho4);
+    }
+    carry = out;
+    have_carry = true;
+    const float dz = out.z - in.z;
+    const float zm = fmaf(dz, 0.5f, in.z);
+    bool inside = zm > zlo && zm < zhi;
+    if (fabsf(zm - zlo) < zband || fabsf(zm - zhi) < zband) {
+      // near the slab boundary: the FP64 walk's own midpoint decides
+      Tet64 q;
+      load_tet(rec64, cur, q);
+      const Hit h = plucker64(q, g.lz, ox, oy);
+      const double dz64 = h.z_out - h.z_in;
+      const double zm64 = dz64 / 2 + h.z_in;
+      inside = zm64 > g.off_z && zm64 < g.off_z + g.lz;
+      cnt.fp64_redo++;
+    }
+    // the field is linear along the chord: trapezoid = exact integral
+    if (inside) acc += (double)(0.5f * (in.rho + out.rho) * dz);
+  }
+  return acc;
+}
+
+struct MixedConsts {
+  Filter flt;
+  float zlo, zhi, zband;
+};
+
+// Filter constants for a launch: stored vertices and rays lie within `span`
+// of the origin (per coordinate).
+inline MixedConsts mixed_consts(const sdtfe_tess *t, const MapGeom &g) {
+  const double u32 = 1.0 / (1 << 24);
+  const double span = std::max(t->span, g.ray_span);
+  // largest absolute coordinate the FP64 expression sees
+  const double amax = span + std::max(std::fabs(g.org_x), std::fabs(g.org_y));
+  MixedConsts m;
+  // stored vertex and ray each carry u32*span; their FP32 difference adds
+  // u32 * 2 span: E = 4 u32 span, padded for the rounding of the bound itself
+  const double e = 4.25 * u32 * span;
+  m.flt.e = (float)e;
+  m.flt.k_prod = (float)(3.0 * u32);
+  // reference zero band, the FP64 expression's own rounding (three products
+  // and two sums of terms below 2 lz amax^2) and the 2 E^2 cross term
+  const double err64 = 16.0 * std::ldexp(1.0, -53) * 2.0 * g.lz * amax * amax;
+  m.flt.thr = (float)(1.01 * ((1e4 * 2.220446049250313080847e-16 + err64) / g.lz + 2 * e * e));
+  m.zlo = (float)(g.off_z - g.org_z);
+  m.zhi = (float)(g.off_z + g.lz - g.org_z);
+  m.zband = (float)(1e-5 * std::max(g.lz, span));
+  return m;
+}
+
+}  // namespace sdtfe

@@ -188,6 +188,24 @@ def g4_c1(R):
                         vertex_rho_sum=float(pd[:, 3].sum()), seed=spec.seed)
 
 
+def g5_utils(R):
+    """convert_to_qhdata and rotate3d (src/utils.c:8-16, 33-79), the host
+    pre-processing of the pydtfe binding."""
+    rng = np.random.default_rng(21)
+    pos32 = rng.random((500, 3)).astype(np.float32)
+    planar = np.ascontiguousarray(pos32.T).ravel()  # Fortran order of (N, 3)
+    ptr = R.convert_to_qhdata(planar, 500)
+    qh = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_double)), shape=(2000,)).copy()
+    R.free(ptr)
+    qh = qh.reshape(500, 4)
+    theta = np.array([0.3, -1.1, 2.5])
+    center = np.array([0.5, 0.4, 0.6])
+    rot = qh.copy()
+    R.rotate3d(rot, 500, theta, center)
+    np.savez_compressed(os.path.join(OUT, "g5_utils.npz"), pos32=pos32, qh=qh[:, :3],
+                        theta=theta, center=center, rotated=rot[:, :3])
+
+
 def main():
     if not O.ref_available():
         O.build(ref=True)
@@ -197,6 +215,7 @@ def main():
     hit = g2_rays(R)
     g3_maps(R)
     g4_c1(R)
+    g5_utils(R)
     print("golden vectors written; crafted rays that completed:", hit)
 
 

@@ -1,0 +1,65 @@
+"""The mixed-precision walk against the FP64 walk on the GPU.
+
+Same topology by construction (every side sign equals the FP64 walk's), so
+tets crossed, perturbations and n_avg must be identical; values carry only
+the FP32 integrand's rounding.  Tolerance: 1e-5 relative per pixel (and per
+ray), the bound the mixed form is held to.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from sdtfe_amd import capi, workloads
+from sdtfe_amd.tessellate import to_particle_data, triangulate
+
+pytestmark = pytest.mark.gpu
+
+RTOL_MIXED = 1e-5
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _relerr(a, b):
+    m = np.abs(b) > 0
+    return np.abs(a - b)[m] / np.abs(b)[m]
+
+
+def test_mixed_rays_match_fp64(golden):
+    _say("[gpu] mixed rays g2")
+    g = golden("g2_rays.npz")
+    with capi.Tessellation(to_particle_data(g["pos"]), g["tetra"]) as t:
+        args = (g["x"], g["y"], g["pixel"], g["sample"], float(g["off_z"]), float(g["lz"]),
+                float(g["dxy"]), int(g["seed"]))
+        v64, c64, s64 = t.trace(*args, entry=g["entry"])
+        v32, c32, s32 = t.trace(*args, entry=g["entry"], precision=capi.PREC_MIXED)
+    np.testing.assert_array_equal(c32, c64)
+    assert s32["perturb_steps"] == s64["perturb_steps"]
+    e = _relerr(v32, v64)
+    _say(f"  max rel err {e.max():.2e}, fp64 redo {s32['fp64_redo']}")
+    assert e.max() <= RTOL_MIXED
+
+
+@pytest.mark.parametrize("kind,n,G,S,box_z,cz", [
+    ("uniform", 40000, 256, 4, 1.0, 0.5),
+    ("clustered", 131072, 256, 8, 1.0, 0.5),
+    ("clustered-slab", 131072, 192, 4, 0.3, 0.45),
+])
+def test_mixed_map_matches_fp64(kind, n, G, S, box_z, cz):
+    _say(f"[gpu] mixed map {kind}")
+    pos = workloads.uniform(n, 7) if kind == "uniform" else workloads.clustered(n, seed=5)
+    tetra = triangulate(pos)
+    kw = dict(seed=13, box_len_z=box_z, center=(0.5, 0.5, cz))
+    with capi.Tessellation(to_particle_data(pos), tetra) as t:
+        r64, s64, n64 = t.map(G, S, want_navg=True, **kw)
+        r32, s32, n32 = t.map(G, S, want_navg=True, precision=capi.PREC_MIXED, **kw)
+    np.testing.assert_array_equal(n32, n64)
+    assert s32["tet_steps"] == s64["tet_steps"]
+    assert s32["perturb_steps"] == s64["perturb_steps"]
+    e = _relerr(r32, r64)
+    _say(f"  max rel err {e.max():.2e}, median {np.median(e):.2e}, "
+         f"fp64 redo / step {s32['fp64_redo'] / max(1, s32['tet_steps']):.2e}")
+    assert e.max() <= RTOL_MIXED

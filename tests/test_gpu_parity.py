@@ -31,6 +31,21 @@ def _rel(a, b):
     return np.max(d / np.maximum(np.abs(b), 1e-300)) if a.size else 0.0
 
 
+def _contains(pd, tri, tet, x, y):
+    """Is (x, y) inside (inclusive) a forward triangle of tet `tet`?"""
+    for row in tri[tri[:, 6] == tet]:
+        a, b, c = pd[row[0], :2], pd[row[1], :2], pd[row[2], :2]
+        ok = True
+        for p, q, o in ((a, b, c), (b, c, a), (c, a, b)):
+            ex, ey = q[0] - p[0], q[1] - p[1]
+            d_ins = (o[0] - p[0]) * ey - (o[1] - p[1]) * ex
+            d_qry = (x - p[0]) * ey - (y - p[1]) * ex
+            ok &= not (d_ins * d_qry < 0.0)
+        if ok:
+            return True
+    return False
+
+
 @pytest.fixture(scope="module", autouse=True)
 def _device():
     n = capi.device_count()
@@ -57,7 +72,16 @@ def test_entry_and_rays_match_reference(golden):
     with capi.Tessellation(pd, g["tetra"]) as t:
         np.testing.assert_array_equal(t.vertex_density(), g["rho"])
         entry = t.locate(g["x"], g["y"])
-        assert (entry == g["entry"]).all(), np.nonzero(entry != g["entry"])[0][:10]
+        n_c = int(g["n_crafted"])
+        # stream samples: the same entry tet as the reference's walk
+        assert (entry[:-n_c] == g["entry"][:-n_c]).all()
+        # crafted rays sit exactly on projected vertices/edges, where several
+        # forward triangles contain the point; any containing one is valid
+        diff = np.nonzero(entry != g["entry"])[0]
+        tri = t.triangles()
+        for i in diff:
+            assert _contains(pd, tri, int(entry[i]), g["x"][i], g["y"][i]), i
+        _say(f"  entry differs on {diff.size} crafted on-edge rays (all valid)")
         val, crossed, st = t.trace(g["x"], g["y"], g["pixel"], g["sample"], float(g["off_z"]),
                                    float(g["lz"]), float(g["dxy"]), int(g["seed"]),
                                    entry=g["entry"])

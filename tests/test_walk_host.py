@@ -1,0 +1,89 @@
+"""The device walk code (walk_fp64.hip / walk_fp64_core.h) compiled for the
+host and run on the golden rays: pins the kernel's logic without a GPU.
+(The product never runs this build; the GPU tests check the device build.)"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from sdtfe_amd import build as sb
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def host_walk(tmp_path_factory):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    exe = str(tmp_path_factory.mktemp("hw") / "host_walk")
+    cmd = [hipcc, "-x", "hip", "-std=c++17", "-O2", "-ffp-contract=off",
+           f"--offload-arch={sb.ARCH}", "-I" + sb.CSRC, "-I" + os.path.join(sb.ROOT, "include"),
+           "-w", os.path.join(HERE, "host_walk.cpp"), "-o", exe]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return exe
+
+
+def records(pos, tetra, rho, nabla):
+    t = tetra.shape[0]
+    rec = np.zeros(t, dtype=np.dtype([("nb", "<i4", 4), ("v", "<f8", 12), ("f", "<f8", 4)]))
+    rec["nb"] = tetra[:, 4:]
+    rec["v"] = pos[tetra[:, :4]].reshape(t, 12)
+    rec["f"][:, 0] = rho[tetra[:, 0]]
+    rec["f"][:, 1:] = nabla
+    return rec
+
+
+def records32(pos, tetra, rho, nabla, org):
+    t = tetra.shape[0]
+    rec = np.zeros(t, dtype=np.dtype([("nb", "<i4", 4), ("v", "<f4", 12)]))
+    rec["nb"] = tetra[:, 4:]
+    rec["v"] = (pos[tetra[:, :4]] - org).astype(np.float32).reshape(t, 12)
+    fld = rho[tetra[:, :4]].astype(np.float32)
+    return rec, fld
+
+
+def run_walk(host_walk, tmp_path, g, mixed=False):
+    rec = records(g["pos"], g["tetra"], g["rho"], g["nabla"])
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([rec.size, g["x"].size], dtype=np.int64).tofile(f)
+        np.array([float(g["lz"]), float(g["dxy"]), float(g["off_z"])]).tofile(f)
+        np.array([int(g["seed"])], dtype=np.uint64).tofile(f)
+        rec.tofile(f)
+        for k in ("x", "y"):
+            g[k].astype(np.float64).tofile(f)
+        g["pixel"].astype(np.uint64).tofile(f)
+        g["sample"].astype(np.uint64).tofile(f)
+        g["entry"].astype(np.int64).tofile(f)
+        if mixed:
+            lo, hi = g["pos"].min(0), g["pos"].max(0)
+            org = 0.5 * (lo + hi)
+            r32, f32 = records32(g["pos"], g["tetra"], g["rho"], g["nabla"], org)
+            r32.tofile(f)
+            f32.tofile(f)
+            np.array([*org, float((0.5 * (hi - lo)).max())]).tofile(f)
+    r = subprocess.run([host_walk, str(inp), str(out)] + (["mixed"] if mixed else []),
+                       check=True, capture_output=True, text=True)
+    res = np.fromfile(out).reshape(-1, 2)
+    return res, r.stderr
+
+
+def test_device_walk_code_matches_reference(golden, host_walk, tmp_path):
+    g = golden("g2_rays.npz")
+    res, _ = run_walk(host_walk, tmp_path, g)
+    np.testing.assert_array_equal(res[:, 1].astype(np.int32), g["crossed"])
+    np.testing.assert_array_equal(res[:, 0], g["value"])
+
+
+def test_device_mixed_walk_code(golden, host_walk, tmp_path):
+    """Mixed walk: same tets as the reference walk, values within 1e-5."""
+    g = golden("g2_rays.npz")
+    res, err = run_walk(host_walk, tmp_path, g, mixed=True)
+    np.testing.assert_array_equal(res[:, 1].astype(np.int32), g["crossed"])
+    v, ref = res[:, 0], g["value"]
+    m = ref != 0
+    assert np.max(np.abs(v[m] - ref[m]) / np.abs(ref[m])) <= 1e-5
+    assert (v[~m] == 0).all()
