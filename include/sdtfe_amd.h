@@ -153,6 +153,67 @@ int sdtfe_map_device(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_devic
                      sdtfe_stats *stats);
 int sdtfe_tess_sync(sdtfe_tess *t);
 
+/* ------------------------------------------------------------ batches */
+
+/* Many maps (e.g. halo cutouts of uneven size, each its own resident
+ * tessellation) in one persistent launch, replayed from a captured hipGraph.
+ * Item i maps tess[i] with params[i] into device buffer rho_device[i]
+ * (grid_dim^2 doubles, accumulated into).  All items share one precision.
+ * lanes_per_pixel (nullable; entries 1, 2, 4, 8 or 16) splits each pixel's
+ * samples over that many lanes; the FP64 map does not depend on it.  Waits
+ * for completion; item_stats (nullable, n entries) and total (nullable)
+ * receive counters; total->t_walk_ms is the launch time. */
+typedef struct sdtfe_batch sdtfe_batch;
+int sdtfe_batch_create(sdtfe_batch **out);
+void sdtfe_batch_destroy(sdtfe_batch *b);
+int sdtfe_batch_run(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
+                    const sdtfe_map_params *params, double *const *rho_device,
+                    const int *lanes_per_pixel, sdtfe_stats *item_stats, sdtfe_stats *total);
+
+/* ------------------------------------------------------------ serving */
+
+/* A map server with continuous batching.  Requests (particles, tets, map
+ * parameters) are copied at submit and admitted in arrival order while their
+ * estimated HBM footprint fits hbm_budget_bytes and fewer than max_active are
+ * resident; a setup thread builds their tessellations while a walk thread
+ * runs persistent launches over the next tiles (at most tile_budget per
+ * launch, shared among resident requests), so requests arriving while others
+ * are on the GPU join the next launch.  Each map is identical alone and in a
+ * mix.  rho (host, grid_dim^2) is accumulated into when the request
+ * completes; sdtfe_server_wait blocks until then.  sdtfe_server_destroy
+ * drains all requests first. */
+typedef struct sdtfe_server sdtfe_server;
+typedef struct {
+  int64_t hbm_budget_bytes;
+  int max_active;
+  int tile_budget;     /* 256-lane tiles per launch */
+  int lanes_per_pixel; /* 1, 2, 4, 8, 16; <= 0: 4 for maps up to 256^2, else 1 */
+  int precision;       /* SDTFE_PREC_*, for every request */
+} sdtfe_server_config;
+typedef struct {
+  int status;
+  double queue_ms;     /* submit -> admitted */
+  double setup_ms;     /* admitted -> first launch */
+  double latency_ms;   /* submit -> map complete */
+  double submit_ms;    /* since server creation */
+  double done_ms;
+  int64_t footprint_bytes;
+  sdtfe_stats stats;
+} sdtfe_request_info;
+typedef struct {
+  int queued, active;
+  int64_t used_bytes;
+  int64_t launches;
+  double walk_ms;      /* summed launch time */
+} sdtfe_server_info;
+int sdtfe_server_create(const sdtfe_server_config *cfg, sdtfe_server **out);
+int sdtfe_server_submit(sdtfe_server *s, const double *particle_data, int n_particles,
+                        const int *tetra_data, int n_tetra, float p_mass,
+                        const sdtfe_map_params *p, double *rho, int64_t *request_id);
+int sdtfe_server_wait(sdtfe_server *s, int64_t request_id, sdtfe_request_info *info);
+int sdtfe_server_get_info(sdtfe_server *s, sdtfe_server_info *info);
+void sdtfe_server_destroy(sdtfe_server *s);
+
 /* Device buffers for callers without another HIP allocator. */
 void *sdtfe_device_alloc(size_t bytes);
 int sdtfe_device_free(void *p);

@@ -46,6 +46,22 @@ class TessInfo(C.Structure):
                 ("t_setup_ms", C.c_double)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int),
+                ("tile_budget", C.c_int), ("lanes_per_pixel", C.c_int), ("precision", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("queue_ms", C.c_double), ("setup_ms", C.c_double),
+                ("latency_ms", C.c_double), ("submit_ms", C.c_double), ("done_ms", C.c_double),
+                ("footprint_bytes", C.c_int64), ("stats", Stats)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("queued", C.c_int), ("active", C.c_int), ("used_bytes", C.c_int64),
+                ("launches", C.c_int64), ("walk_ms", C.c_double)]
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
@@ -70,6 +86,17 @@ _SIGS = {
     "sdtfe_map": (C.c_int, [_vp, C.POINTER(MapParams), _dp, _vp, C.POINTER(Stats)]),
     "sdtfe_map_device": (C.c_int, [_vp, C.POINTER(MapParams), _vp, C.POINTER(Stats)]),
     "sdtfe_tess_sync": (C.c_int, [_vp]),
+    "sdtfe_batch_create": (C.c_int, [C.POINTER(_vp)]),
+    "sdtfe_batch_destroy": (None, [_vp]),
+    "sdtfe_batch_run": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(MapParams),
+                                  C.POINTER(_vp), C.POINTER(C.c_int), C.POINTER(Stats),
+                                  C.POINTER(Stats)]),
+    "sdtfe_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    "sdtfe_server_submit": (C.c_int, [_vp, _dp, C.c_int, _ip, C.c_int, C.c_float,
+                                      C.POINTER(MapParams), _dp, C.POINTER(C.c_int64)]),
+    "sdtfe_server_wait": (C.c_int, [_vp, C.c_int64, C.POINTER(RequestInfo)]),
+    "sdtfe_server_get_info": (C.c_int, [_vp, C.POINTER(ServerInfo)]),
+    "sdtfe_server_destroy": (None, [_vp]),
     "sdtfe_device_alloc": (_vp, [C.c_size_t]),
     "sdtfe_device_free": (C.c_int, [_vp]),
     "sdtfe_memcpy_d2h": (C.c_int, [_vp, _vp, C.c_size_t]),
@@ -264,6 +291,103 @@ class Tessellation:
 
     def sync(self):
         check(lib().sdtfe_tess_sync(self.h))
+
+
+class Batch:
+    """Many maps in one persistent, graph-replayed launch (sdtfe_batch)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().sdtfe_batch_create(C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sdtfe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, items, lanes_per_pixel=None):
+        """items: list of (Tessellation, MapParams, DeviceBuffer).  Returns
+        (per-item stats, total stats); maps accumulate into the buffers."""
+        n = len(items)
+        tess = (C.c_void_p * n)(*[it[0].h.value for it in items])
+        params = (MapParams * n)(*[it[1] for it in items])
+        outs = (C.c_void_p * n)(*[it[2].ptr for it in items])
+        lpp = None
+        if lanes_per_pixel is not None:
+            lpp = (C.c_int * n)(*[int(v) for v in lanes_per_pixel])
+        st = (Stats * n)()
+        tot = Stats()
+        check(lib().sdtfe_batch_run(self.h, n, tess, params, outs, lpp, st, C.byref(tot)))
+        return [x.as_dict() for x in st], tot.as_dict()
+
+
+class Server:
+    """Map server with continuous batching (sdtfe_server)."""
+
+    def __init__(self, hbm_budget_bytes=64 << 30, max_active=64, tile_budget=4096,
+                 lanes_per_pixel=0, precision=PREC_FP64):
+        cfg = ServerConfig(int(hbm_budget_bytes), int(max_active), int(tile_budget),
+                           int(lanes_per_pixel), int(precision))
+        h = C.c_void_p()
+        check(lib().sdtfe_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.precision = precision
+        self._out = {}  # request id -> output array kept alive until waited
+
+    def submit(self, particle_data, tetra_data, params: MapParams, p_mass=1.0, rho=None):
+        pd = _c64(particle_data)
+        tet = _c64(tetra_data, np.int32)
+        npix = params.grid_dim * params.grid_dim
+        out = np.zeros(npix) if rho is None else rho
+        rid = C.c_int64()
+        check(lib().sdtfe_server_submit(self.h, pd, pd.shape[0], tet, tet.shape[0], p_mass,
+                                        C.byref(params), out, C.byref(rid)))
+        self._out[rid.value] = out
+        return rid.value
+
+    def wait(self, rid):
+        info = RequestInfo()
+        rc = lib().sdtfe_server_wait(self.h, rid, C.byref(info))
+        out = self._out.pop(rid)
+        check(rc)
+        d = {n: getattr(info, n) for n, _ in info._fields_ if n != "stats"}
+        d["stats"] = info.stats.as_dict()
+        return out, d
+
+    def info(self):
+        i = ServerInfo()
+        check(lib().sdtfe_server_get_info(self.h, C.byref(i)))
+        return {n: getattr(i, n) for n, _ in i._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sdtfe_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def compute_density(particle_data, tetra_data, grid_dim, box_len, box_len_z, rho, p_mass,

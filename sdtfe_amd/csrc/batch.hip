@@ -1,0 +1,429 @@
+// batch.hip -- many maps (cutouts of uneven size) in one persistent launch.
+//
+// Work items are 256-lane tiles of pixels.  Item i's map is cut into tiles
+// of (256 / L_i) pixels, where L_i ("lanes per pixel") splits each pixel's
+// samples over L_i lanes: L = 1 gives one pixel per lane (16x16 pixels per
+// tile), L = 4 gives 8x8 pixels whose samples are spread over 4 lanes each
+// (more parallelism for small maps and short columns).  Per-sample values
+// land in LDS and one lane per pixel sums them in sample order, so the FP64
+// map is bit-identical to the one-pixel-per-lane launch.
+//
+// The tile space is item-major and cut into eight contiguous shares, one per
+// XCD; a block (dealt round-robin to the XCDs) claims tiles of its own
+// share from that share's counter, so neighbouring tiles run on one L2, and
+// takes tiles from the other shares when its own is exhausted.  Every wave
+// leaves once all counters pass their ends.
+//
+// The launch (counter reset kernel + persistent kernel) is captured once in
+// a hipGraph and replayed; item descriptors are copied in before each
+// replay, so the graph stays valid as long as the buffers do.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "sdtfe_host.h"
+#include "walk_mixed_core.h"
+
+namespace sdtfe {
+
+constexpr int kXcd = 8;
+constexpr int kLdsBytes = 32 * 1024;
+
+struct BatchItem {
+  const Rec64 *rec64;
+  const Rec32 *rec32;
+  const Field32 *fld;
+  EntryGrid grid;
+  MapGeom geom;
+  MixedConsts mc;
+  double *out;
+  int lpp;          // lanes per pixel
+  int tile_w;       // pixels per tile row
+  int tile_h;
+  int tiles_x;
+  int first_tile;   // prefix over items
+  int n_tiles;      // tiles of this item in this launch
+  int tile_offset;  // first tile of the item's map covered by this launch
+};
+
+struct BatchHdr {
+  int n_items;
+  int total_tiles;
+  int share[kXcd + 1];  // tile-space cut points
+};
+
+// Wave sub-tile shape for L lanes per pixel: 64/L pixels as bw x bh.
+SDTFE_HD inline void wave_block(int lpp, int &bw, int &bh) {
+  switch (lpp) {
+    case 1: bw = 8; bh = 8; break;
+    case 2: bw = 8; bh = 4; break;
+    case 4: bw = 4; bh = 4; break;
+    case 8: bw = 4; bh = 2; break;
+    default: bw = 2; bh = 2; break;  // 16
+  }
+}
+
+__global__ void batch_reset_kernel(const BatchHdr *hdr, int *counters,
+                                   unsigned long long *item_stats, int n_stats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kXcd) counters[i] = hdr->share[i];
+  for (int k = i; k < n_stats; k += gridDim.x * blockDim.x) item_stats[k] = 0ull;
+}
+
+template <bool kMixed>
+__device__ inline double walk_one(const BatchItem &it, int entry, double x, double y,
+                                  uint64_t key, int &crossed, LaneCounters &cnt) {
+  if (kMixed)
+    return walk_mixed(it.rec32, it.fld, it.rec64, entry, x, y, it.geom, it.mc.flt, it.mc.zlo,
+                      it.mc.zhi, it.mc.zband, key, crossed, cnt);
+  return walk64(it.rec64, entry, x, y, it.geom, key, crossed, cnt);
+}
+
+template <bool kMixed>
+__global__ __launch_bounds__(256) void batch_kernel(const BatchItem *__restrict__ items,
+                                                    const BatchHdr *__restrict__ hdr,
+                                                    int *counters,
+                                                    unsigned long long *item_stats) {
+  extern __shared__ double lds[];  // per-sample values for lanes-per-pixel > 1
+  __shared__ int s_tile;
+  const int home = blockIdx.x % kXcd;
+  const int n_items = hdr->n_items;
+  int xcd = home, tried = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int t = -1;
+      while (tried < kXcd) {
+        const int c = atomicAdd(counters + xcd, 1);
+        if (c < hdr->share[xcd + 1]) {
+          t = c;
+          break;
+        }
+        xcd = (xcd + 1) % kXcd;
+        ++tried;
+      }
+      s_tile = t;
+    }
+    __syncthreads();
+    const int tile = s_tile;
+    __syncthreads();
+    if (tile < 0) break;
+    // item holding this tile (items sorted by first_tile)
+    int lo = 0, hi = n_items - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (items[mid].first_tile <= tile) lo = mid; else hi = mid - 1;
+    }
+    const BatchItem &it = items[lo];
+    const MapGeom &g = it.geom;
+    const int lt = tile - it.first_tile + it.tile_offset;
+    const int L = it.lpp;
+    int bw, bh;
+    wave_block(L, bw, bh);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = lane / L, j = lane % L;  // pixel within the wave block, sample slice
+    const int px = (lt % it.tiles_x) * it.tile_w + (w & 1) * bw + q % bw;
+    const int py = (lt / it.tiles_x) * it.tile_h + (w >> 1) * bh + q / bw;
+    const bool active = px < g.grid && py < g.grid;
+    const uint64_t pix = (uint64_t)py * (uint64_t)g.grid + (uint64_t)px;
+    LaneCounters cnt;
+    double sum = 0.0;
+    int n_ok = 0;
+    const int slot = threadIdx.x / L;  // pixel slot in the block
+    if (active) {
+      for (int s = j; s < g.samples; s += L) {
+        const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
+        double xt, yt;
+        sample_point(g, px, py, key, xt, yt);
+        cnt.samples++;
+        const int entry = locate_entry(it.grid, xt, yt);
+        double v = 0.0;
+        int crossed = 0;
+        if (entry < 0) {
+          cnt.missed++;
+        } else {
+          v = walk_one<kMixed>(it, entry, xt, yt, key, crossed, cnt);
+          cnt.tet_steps += (unsigned)crossed;
+          if (crossed == 0) cnt.failed++;
+        }
+        if (L == 1) {
+          sum += v;  // a miss or a failed walk adds 0 (src/dtfe.c:626-636)
+        } else {
+          lds[slot * g.samples + s] = v;
+        }
+        n_ok += crossed > 0;
+      }
+    }
+    if (L > 1) {
+      // n_avg: lanes of one pixel are adjacent within the wave
+      for (int off = 1; off < L; off <<= 1) n_ok += __shfl_xor(n_ok, off, 64);
+      __syncthreads();
+      if (active && j == 0)
+        for (int s = 0; s < g.samples; ++s) sum += lds[slot * g.samples + s];
+    }
+    if (active && j == 0) it.out[pix] += n_ok > 0 ? sum / n_ok : 0.0;
+    cnt.flush(item_stats + (size_t)lo * ST_COUNT);
+    __syncthreads();
+  }
+}
+
+}  // namespace sdtfe
+
+// ------------------------------------------------------------ host side
+
+struct sdtfe_batch {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  sdtfe::BatchItem *d_items = nullptr;
+  sdtfe::BatchHdr *d_hdr = nullptr;
+  int *d_counters = nullptr;
+  unsigned long long *d_stats = nullptr;
+  sdtfe::BatchItem *h_items = nullptr;  // pinned
+  sdtfe::BatchHdr *h_hdr = nullptr;     // pinned
+  int cap = 0;
+  // captured launch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_mixed = false;
+  int graph_blocks = 0;
+  size_t graph_lds = 0;
+  int blocks_fp64 = 0, blocks_mixed = 0;
+};
+
+namespace sdtfe {
+namespace {
+
+void drop_graph(sdtfe_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int ensure_cap(sdtfe_batch *b, int n) {
+  if (n <= b->cap) return 0;
+  drop_graph(b);
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_stats) (void)hipFree(b->d_stats);
+  if (b->h_items) (void)hipHostFree(b->h_items);
+  b->d_items = nullptr;
+  b->d_stats = nullptr;
+  b->h_items = nullptr;
+  b->cap = 0;
+  const int cap = std::max(n, 64);
+  if (hipMalloc((void **)&b->d_items, sizeof(BatchItem) * cap) != hipSuccess ||
+      hipMalloc((void **)&b->d_stats, sizeof(unsigned long long) * ST_COUNT * cap) !=
+          hipSuccess ||
+      hipHostMalloc((void **)&b->h_items, sizeof(BatchItem) * cap, hipHostMallocDefault) !=
+          hipSuccess)
+    return fail(SDTFE_ENOMEM, "batch buffers");
+  b->cap = cap;
+  return 0;
+}
+
+template <bool kMixed>
+int blocks_for(sdtfe_batch *, size_t lds) {
+  int per_cu = 0;
+  SDTFE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<kMixed>, 256,
+                                                         lds));
+  int dev = 0;
+  SDTFE_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  SDTFE_HIP(hipGetDeviceProperties(&prop, dev));
+  return std::max(1, per_cu) * prop.multiProcessorCount;
+}
+
+// Capture reset + persistent kernel on the batch's stream.
+template <bool kMixed>
+int capture(sdtfe_batch *b, int blocks, size_t lds) {
+  drop_graph(b);
+  SDTFE_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  batch_reset_kernel<<<64, 256, 0, b->stream>>>(b->d_hdr, b->d_counters, b->d_stats,
+                                                ST_COUNT * b->cap);
+  batch_kernel<kMixed><<<blocks, 256, lds, b->stream>>>(b->d_items, b->d_hdr, b->d_counters,
+                                                         b->d_stats);
+  hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+  if (e != hipSuccess) return hip_check(e, "hipStreamEndCapture");
+  SDTFE_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->graph_mixed = kMixed;
+  b->graph_blocks = blocks;
+  b->graph_lds = lds;
+  return 0;
+}
+
+}  // namespace
+
+}  // namespace sdtfe
+
+namespace sdtfe {
+
+int batch_tiles(const sdtfe_map_params &p, int lanes_per_pixel) {
+  int L = lanes_per_pixel;
+  if (L != 1 && L != 2 && L != 4 && L != 8 && L != 16) L = 1;
+  while (L > 1 && (size_t)(256 / L) * (size_t)std::max(1, p.n_mc_samp) * 8 > (size_t)kLdsBytes)
+    L >>= 1;
+  int bw, bh;
+  wave_block(L, bw, bh);
+  const int tx = (p.grid_dim + 2 * bw - 1) / (2 * bw), ty = (p.grid_dim + 2 * bh - 1) / (2 * bh);
+  return tx * ty;
+}
+
+int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
+                     const sdtfe_map_params *params, double *const *rho_device,
+                     const int *lanes_per_pixel, const int *tile_begin, const int *tile_count,
+                     sdtfe_stats *item_stats, sdtfe_stats *total) {
+  if (!b || n < 0 || (n && (!tess || !params || !rho_device)))
+    return fail(SDTFE_EINVAL, "bad argument");
+  if (n == 0) return 0;
+  const int prec = params[0].precision;
+  for (int i = 0; i < n; ++i) {
+    if (!tess[i] || !rho_device[i]) return fail(SDTFE_EINVAL, "null item");
+    if (params[i].precision != prec) return fail(SDTFE_EINVAL, "mixed precisions in one batch");
+    if (params[i].grid_dim <= 0 || params[i].grid_dim > 65536 || params[i].n_mc_samp < 0 ||
+        !(params[i].box_len > 0) || !(params[i].box_len_z > 0))
+      return fail(SDTFE_EINVAL, "bad map parameters");
+  }
+  if (prec != SDTFE_PREC_FP64 && prec != SDTFE_PREC_MIXED)
+    return fail(SDTFE_EINVAL, "unknown precision");
+  int rc = ensure_cap(b, n);
+  if (rc) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  // tessellations' setup must be complete before the batch stream reads them
+  for (int i = 0; i < n; ++i) SDTFE_HIP(hipStreamSynchronize(tess[i]->stream));
+
+  int tiles = 0;
+  for (int i = 0; i < n; ++i) {
+    const sdtfe_map_params &p = params[i];
+    BatchItem &it = b->h_items[i];
+    std::memset(&it, 0, sizeof it);
+    it.rec64 = tess[i]->rec64;
+    it.rec32 = tess[i]->rec32;
+    it.fld = tess[i]->fld32;
+    it.grid = tess[i]->grid;
+    it.geom = make_geom_pub(tess[i], p);
+    it.mc = mixed_consts(tess[i], it.geom);
+    it.out = rho_device[i];
+    int L = lanes_per_pixel ? lanes_per_pixel[i] : 1;
+    if (L != 1 && L != 2 && L != 4 && L != 8 && L != 16) L = 1;
+    // per-sample values of a tile must fit in LDS
+    while (L > 1 && (size_t)(256 / L) * (size_t)std::max(1, p.n_mc_samp) * 8 > (size_t)kLdsBytes)
+      L >>= 1;
+    int bw, bh;
+    wave_block(L, bw, bh);
+    it.lpp = L;
+    it.tile_w = 2 * bw;
+    it.tile_h = 2 * bh;
+    it.tiles_x = (p.grid_dim + it.tile_w - 1) / it.tile_w;
+    const int tiles_y = (p.grid_dim + it.tile_h - 1) / it.tile_h;
+    const int all = it.tiles_x * tiles_y;
+    const int b0 = tile_begin ? std::min(std::max(tile_begin[i], 0), all) : 0;
+    const int nt = tile_count ? std::min(std::max(tile_count[i], 0), all - b0) : all - b0;
+    it.tile_offset = b0;
+    it.n_tiles = nt;
+    it.first_tile = tiles;
+    tiles += nt;
+  }
+  BatchHdr &h = *b->h_hdr;
+  h.n_items = n;
+  h.total_tiles = tiles;
+  for (int x = 0; x <= kXcd; ++x) h.share[x] = (int)((int64_t)tiles * x / kXcd);
+
+  const bool mixed = prec == SDTFE_PREC_MIXED;
+  int &blocks = mixed ? b->blocks_mixed : b->blocks_fp64;
+  if (!blocks) blocks = mixed ? blocks_for<true>(b, kLdsBytes) : blocks_for<false>(b, kLdsBytes);
+  if (blocks <= 0) return fail(SDTFE_EHIP, "occupancy query failed");
+  // one LDS size for every launch, so one captured graph serves every batch
+  if (!b->exec || b->graph_mixed != mixed || b->graph_blocks != blocks) {
+    rc = mixed ? capture<true>(b, blocks, kLdsBytes) : capture<false>(b, blocks, kLdsBytes);
+    if (rc) return rc;
+  }
+  SDTFE_HIP(hipMemcpyAsync(b->d_items, b->h_items, sizeof(BatchItem) * n, hipMemcpyHostToDevice,
+                           b->stream));
+  SDTFE_HIP(hipMemcpyAsync(b->d_hdr, b->h_hdr, sizeof(BatchHdr), hipMemcpyHostToDevice,
+                           b->stream));
+  SDTFE_HIP(hipEventRecord(b->ev0, b->stream));
+  SDTFE_HIP(hipGraphLaunch(b->exec, b->stream));
+  SDTFE_HIP(hipEventRecord(b->ev1, b->stream));
+  SDTFE_HIP(hipEventSynchronize(b->ev1));
+  float ms = 0;
+  SDTFE_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  std::vector<unsigned long long> st((size_t)n * ST_COUNT);
+  SDTFE_HIP(hipMemcpy(st.data(), b->d_stats, sizeof(unsigned long long) * st.size(),
+                      hipMemcpyDeviceToHost));
+  sdtfe_stats sum{};
+  for (int i = 0; i < n; ++i) {
+    sdtfe_stats s{};
+    fill_stats_pub(st.data() + (size_t)i * ST_COUNT, &s);
+    if (item_stats) item_stats[i] = s;
+    sum.samples += s.samples;
+    sum.missed += s.missed;
+    sum.failed += s.failed;
+    sum.no_intersection += s.no_intersection;
+    sum.perturb_exhausted += s.perturb_exhausted;
+    sum.no_isect_perturb += s.no_isect_perturb;
+    sum.perturb_steps += s.perturb_steps;
+    sum.tet_steps += s.tet_steps;
+    sum.step_cap += s.step_cap;
+    sum.fp64_redo += s.fp64_redo;
+  }
+  sum.t_walk_ms = ms;
+  sum.t_total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (total) *total = sum;
+  return 0;
+}
+
+}  // namespace sdtfe
+
+using namespace sdtfe;
+
+extern "C" {
+
+int sdtfe_batch_create(sdtfe_batch **out) {
+  if (!out) return fail(SDTFE_EINVAL, "null output handle");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(SDTFE_ENODEV, "no HIP device visible");
+  sdtfe_batch *b = new sdtfe_batch();
+  int rc = hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+  if (!rc) rc = hip_check(hipEventCreate(&b->ev0), "event");
+  if (!rc) rc = hip_check(hipEventCreate(&b->ev1), "event");
+  if (!rc && (hipMalloc((void **)&b->d_hdr, sizeof(BatchHdr)) != hipSuccess ||
+              hipMalloc((void **)&b->d_counters, sizeof(int) * kXcd) != hipSuccess ||
+              hipHostMalloc((void **)&b->h_hdr, sizeof(BatchHdr), hipHostMallocDefault) !=
+                  hipSuccess))
+    rc = fail(SDTFE_ENOMEM, "batch header");
+  if (!rc) rc = ensure_cap(b, 64);
+  if (rc) {
+    sdtfe_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+void sdtfe_batch_destroy(sdtfe_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_stats) (void)hipFree(b->d_stats);
+  if (b->d_hdr) (void)hipFree(b->d_hdr);
+  if (b->d_counters) (void)hipFree(b->d_counters);
+  if (b->h_items) (void)hipHostFree(b->h_items);
+  if (b->h_hdr) (void)hipHostFree(b->h_hdr);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int sdtfe_batch_run(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
+                    const sdtfe_map_params *params, double *const *rho_device,
+                    const int *lanes_per_pixel, sdtfe_stats *item_stats, sdtfe_stats *total) {
+  return sdtfe::batch_run_ranges(b, n, tess, params, rho_device, lanes_per_pixel, nullptr,
+                                 nullptr, item_stats, total);
+}
+
+}  // extern "C"

@@ -47,8 +47,10 @@ double ms_since(Clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
 }
 
+}  // namespace
+
 // Map geometry exactly as src/dtfe.c:579-589 computes it.
-MapGeom make_geom(const sdtfe_tess *t, const sdtfe_map_params &p) {
+MapGeom make_geom_pub(const sdtfe_tess *t, const sdtfe_map_params &p) {
   MapGeom g{};
   g.dxy = p.box_len / p.grid_dim;
   g.off_x = -(p.box_len / 2.0) + p.center[0];
@@ -71,6 +73,21 @@ MapGeom make_geom(const sdtfe_tess *t, const sdtfe_map_params &p) {
                         std::max(std::fabs(lo_y - g.org_y), std::fabs(hi_y - g.org_y)));
   return g;
 }
+
+void fill_stats_pub(const unsigned long long *c, sdtfe_stats *s) {
+  s->samples = (int64_t)c[ST_SAMPLES];
+  s->missed = (int64_t)c[ST_MISSED];
+  s->failed = (int64_t)c[ST_FAILED];
+  s->no_intersection = (int64_t)c[ST_NO_ISECT];
+  s->perturb_exhausted = (int64_t)c[ST_PERT_EXHAUSTED];
+  s->no_isect_perturb = (int64_t)c[ST_NO_ISECT_PERT];
+  s->perturb_steps = (int64_t)c[ST_PERT_STEPS];
+  s->tet_steps = (int64_t)c[ST_TET_STEPS];
+  s->step_cap = (int64_t)c[ST_STEP_CAP];
+  s->fp64_redo = (int64_t)c[ST_FP64_REDO];
+}
+
+namespace {
 
 int check_params(const sdtfe_map_params *p) {
   if (!p) return fail(SDTFE_EINVAL, "null map parameters");
@@ -98,24 +115,11 @@ int ensure_map_buf(sdtfe_tess *t, size_t npix) {
   return 0;
 }
 
-void fill_stats(const unsigned long long *c, sdtfe_stats *s) {
-  s->samples = (int64_t)c[ST_SAMPLES];
-  s->missed = (int64_t)c[ST_MISSED];
-  s->failed = (int64_t)c[ST_FAILED];
-  s->no_intersection = (int64_t)c[ST_NO_ISECT];
-  s->perturb_exhausted = (int64_t)c[ST_PERT_EXHAUSTED];
-  s->no_isect_perturb = (int64_t)c[ST_NO_ISECT_PERT];
-  s->perturb_steps = (int64_t)c[ST_PERT_STEPS];
-  s->tet_steps = (int64_t)c[ST_TET_STEPS];
-  s->step_cap = (int64_t)c[ST_STEP_CAP];
-  s->fp64_redo = (int64_t)c[ST_FP64_REDO];
-}
-
 int read_stats(sdtfe_tess *t, sdtfe_stats *s) {
   unsigned long long c[ST_COUNT];
   SDTFE_HIP(hipMemcpyAsync(c, t->stats, sizeof c, hipMemcpyDeviceToHost, t->stream));
   SDTFE_HIP(hipStreamSynchronize(t->stream));
-  if (s) fill_stats(c, s);
+  if (s) fill_stats_pub(c, s);
   return 0;
 }
 
@@ -180,7 +184,7 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
 
 int run_map(sdtfe_tess *t, const sdtfe_map_params *p, double *out, int *navg, int accumulate,
             double *t_walk_ms) {
-  const MapGeom g = make_geom(t, *p);
+  const MapGeom g = make_geom_pub(t, *p);
   SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
   SDTFE_HIP(hipEventRecord(t->ev0, t->stream));
   if (p->precision == SDTFE_PREC_MIXED)

@@ -70,6 +70,17 @@ struct sdtfe_tess {
 };
 
 namespace sdtfe {
+// capi.cpp
+MapGeom make_geom_pub(const sdtfe_tess *t, const sdtfe_map_params &p);
+void fill_stats_pub(const unsigned long long *c, sdtfe_stats *s);
+// batch.hip: tiles of one map in a batch, and a batched launch over tile
+// ranges [tile_begin[i], tile_begin[i] + tile_count[i]) of each item (null
+// ranges: whole maps).
+int batch_tiles(const sdtfe_map_params &p, int lanes_per_pixel);
+int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
+                     const sdtfe_map_params *params, double *const *rho_device,
+                     const int *lanes_per_pixel, const int *tile_begin, const int *tile_count,
+                     sdtfe_stats *item_stats, sdtfe_stats *total);
 // field.hip
 int build_field(sdtfe_tess *t, const double *particle_data, const int *tetra_data);
 int build_entry_grid(sdtfe_tess *t);

@@ -1,0 +1,370 @@
+// server.cpp -- a map-serving loop with continuous batching (host C++ over
+// the batched launch of batch.hip).
+//
+//   submit()      copies a request (particles, tets, map parameters) into
+//                 host memory owned by the server and queues it.
+//   setup thread  admits queued requests in arrival order while their HBM
+//                 footprint fits the budget and fewer than max_active are
+//                 resident, uploads them and builds their tessellations
+//                 (sdtfe_tess_create) on their own streams.
+//   walk thread   repeatedly launches one persistent batch over the next
+//                 tiles of every resident request (at most tile_budget tiles
+//                 per launch, shared round-robin), so a request admitted
+//                 while others are on the GPU joins the next launch; a
+//                 request whose tiles are all done is copied out, freed and
+//                 its waiters woken.
+//
+// Every pixel is computed by the same code with the same stream whatever
+// else shares the launch, so a map is identical alone and in a mix.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "sdtfe_host.h"
+
+namespace sdtfe {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  int64_t id = 0;
+  std::vector<double> pd;
+  std::vector<int> tets;
+  float p_mass = 1.f;
+  sdtfe_map_params params{};
+  double *out = nullptr;  // caller's host map, accumulated into
+  int64_t footprint = 0;
+  sdtfe_tess *tess = nullptr;
+  double *d_map = nullptr;
+  int n_tiles = 0, next_tile = 0, lpp = 1;
+  Clock::time_point t_submit, t_admit, t_start, t_done;
+  bool started = false, done = false;
+  int status = 0;
+  sdtfe_stats stats{};
+};
+
+int64_t estimate_footprint(int n, int nt, int grid) {
+  // resident: pos4 32 B/vertex; per tet: row 32, volume 8, gradient 24,
+  // records 144 + 64 + 16; map 8 B/pixel; setup scratch ~ 16 B/tet + 12 B/vertex
+  return (int64_t)n * 44 + (int64_t)nt * 304 + (int64_t)grid * grid * 8 + (1 << 20);
+}
+
+}  // namespace
+}  // namespace sdtfe
+
+struct sdtfe_server {
+  sdtfe_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv_setup, cv_walk, cv_done;
+  std::deque<std::shared_ptr<sdtfe::Request>> queued;
+  std::vector<std::shared_ptr<sdtfe::Request>> active;  // resident, tiles pending
+  std::map<int64_t, std::shared_ptr<sdtfe::Request>> all;
+  int64_t next_id = 1;
+  int64_t used_bytes = 0;
+  int n_setup = 0;  // being set up (counted in used_bytes)
+  bool stop = false;
+  sdtfe_batch *batch = nullptr;
+  std::thread setup_thr, walk_thr;
+  sdtfe::Clock::time_point t0;
+  int64_t launches = 0;
+  double walk_ms = 0;
+};
+
+namespace sdtfe {
+namespace {
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+void add_stats(sdtfe_stats &a, const sdtfe_stats &s) {
+  a.samples += s.samples;
+  a.missed += s.missed;
+  a.failed += s.failed;
+  a.no_intersection += s.no_intersection;
+  a.perturb_exhausted += s.perturb_exhausted;
+  a.no_isect_perturb += s.no_isect_perturb;
+  a.perturb_steps += s.perturb_steps;
+  a.tet_steps += s.tet_steps;
+  a.step_cap += s.step_cap;
+  a.fp64_redo += s.fp64_redo;
+}
+
+void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status) {
+  // caller holds no lock; device work of r is complete
+  if (status == 0 && r->d_map) {
+    const size_t npix = (size_t)r->params.grid_dim * r->params.grid_dim;
+    std::vector<double> m(npix);
+    if (hipMemcpy(m.data(), r->d_map, sizeof(double) * npix, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      status = SDTFE_EHIP;
+    else
+      for (size_t i = 0; i < npix; ++i) r->out[i] += m[i];
+  }
+  if (r->d_map) (void)hipFree(r->d_map);
+  r->d_map = nullptr;
+  sdtfe_tess_destroy(r->tess);
+  r->tess = nullptr;
+  std::vector<double>().swap(r->pd);
+  std::vector<int>().swap(r->tets);
+  std::lock_guard<std::mutex> g(sv->mu);
+  r->status = status;
+  r->done = true;
+  r->t_done = Clock::now();
+  sv->used_bytes -= r->footprint;
+  sv->cv_done.notify_all();
+  sv->cv_setup.notify_all();
+}
+
+void setup_loop(sdtfe_server *sv) {
+  for (;;) {
+    std::shared_ptr<Request> r;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_setup.wait(lk, [&] {
+        if (sv->stop) return true;
+        if (sv->queued.empty()) return false;
+        const auto &q = sv->queued.front();
+        const int resident = (int)sv->active.size() + sv->n_setup;
+        const bool fits = sv->used_bytes + q->footprint <= sv->cfg.hbm_budget_bytes ||
+                          (resident == 0);  // a lone request is always admitted
+        return resident < sv->cfg.max_active && fits;
+      });
+      if (sv->stop) return;
+      r = sv->queued.front();
+      sv->queued.pop_front();
+      sv->used_bytes += r->footprint;
+      sv->n_setup++;
+      r->t_admit = Clock::now();
+    }
+    int rc = sdtfe_tess_create(r->pd.data(), (int)(r->pd.size() / 4), r->tets.data(),
+                               (int)(r->tets.size() / 8), r->p_mass, &r->tess);
+    const size_t npix = (size_t)r->params.grid_dim * r->params.grid_dim;
+    if (rc == 0 && hipMalloc((void **)&r->d_map, sizeof(double) * npix) != hipSuccess)
+      rc = fail(SDTFE_ENOMEM, "map buffer");
+    if (rc == 0 && hipMemset(r->d_map, 0, sizeof(double) * npix) != hipSuccess) rc = SDTFE_EHIP;
+    if (rc == 0) {
+      r->lpp = sv->cfg.lanes_per_pixel > 0 ? sv->cfg.lanes_per_pixel
+                                           : (r->params.grid_dim <= 256 ? 4 : 1);
+      r->n_tiles = batch_tiles(r->params, r->lpp);
+    }
+    {
+      std::lock_guard<std::mutex> g(sv->mu);
+      sv->n_setup--;
+    }
+    if (rc) {
+      finish(sv, r, rc);
+      continue;
+    }
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->active.push_back(r);
+    sv->cv_walk.notify_all();
+  }
+}
+
+void walk_loop(sdtfe_server *sv) {
+  size_t rr = 0;  // round-robin start
+  for (;;) {
+    std::vector<std::shared_ptr<Request>> items;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_walk.wait(lk, [&] { return sv->stop || !sv->active.empty(); });
+      if (sv->stop) return;
+      items = sv->active;
+    }
+    // share the tile budget: every request gets an equal slice, leftovers
+    // go round-robin
+    const int n = (int)items.size();
+    std::vector<int> begin(n), count(n, 0);
+    int budget = std::max(sv->cfg.tile_budget, n);
+    int remaining = 0;
+    for (int i = 0; i < n; ++i) remaining += items[i]->n_tiles - items[i]->next_tile;
+    budget = std::min(budget, remaining);
+    const int fair = std::max(1, budget / n);
+    int given = 0;
+    for (int k = 0; k < n; ++k) {
+      const int i = (int)((rr + k) % n);
+      const int left = items[i]->n_tiles - items[i]->next_tile;
+      count[i] = std::min(left, fair);
+      given += count[i];
+    }
+    for (int k = 0; k < n && given < budget; ++k) {
+      const int i = (int)((rr + k) % n);
+      const int extra = std::min(items[i]->n_tiles - items[i]->next_tile - count[i],
+                                 budget - given);
+      count[i] += extra;
+      given += extra;
+    }
+    rr = (rr + 1) % std::max(n, 1);
+    std::vector<sdtfe_tess *> ts;
+    std::vector<sdtfe_map_params> ps;
+    std::vector<double *> outs;
+    std::vector<int> lp, b0, cnt;
+    std::vector<std::shared_ptr<Request>> in_launch;
+    const auto now = Clock::now();
+    for (int i = 0; i < n; ++i) {
+      if (count[i] == 0) continue;
+      Request &r = *items[i];
+      if (!r.started) {
+        r.started = true;
+        r.t_start = now;
+      }
+      ts.push_back(r.tess);
+      ps.push_back(r.params);
+      outs.push_back(r.d_map);
+      lp.push_back(r.lpp);
+      b0.push_back(r.next_tile);
+      cnt.push_back(count[i]);
+      in_launch.push_back(items[i]);
+    }
+    std::vector<sdtfe_stats> st(ts.size());
+    sdtfe_stats tot{};
+    const int rc = ts.empty() ? 0
+                              : batch_run_ranges(sv->batch, (int)ts.size(), ts.data(), ps.data(),
+                                                 outs.data(), lp.data(), b0.data(), cnt.data(),
+                                                 st.data(), &tot);
+    std::vector<std::shared_ptr<Request>> finished;
+    {
+      std::lock_guard<std::mutex> g(sv->mu);
+      sv->launches++;
+      sv->walk_ms += tot.t_walk_ms;
+      for (size_t k = 0; k < in_launch.size(); ++k) {
+        Request &r = *in_launch[k];
+        add_stats(r.stats, st[k]);
+        r.next_tile += cnt[k];
+        if (rc || r.next_tile >= r.n_tiles) finished.push_back(in_launch[k]);
+      }
+      for (auto &f : finished)
+        sv->active.erase(std::remove(sv->active.begin(), sv->active.end(), f), sv->active.end());
+      // requests with no tiles at all
+      for (auto it = sv->active.begin(); it != sv->active.end();) {
+        if ((*it)->n_tiles == 0) {
+          finished.push_back(*it);
+          it = sv->active.erase(it);
+        } else {
+          ++it;
+        }
+      }
+    }
+    for (auto &f : finished) finish(sv, f, rc);
+  }
+}
+
+}  // namespace
+}  // namespace sdtfe
+
+using namespace sdtfe;
+
+extern "C" {
+
+int sdtfe_server_create(const sdtfe_server_config *cfg, sdtfe_server **out) {
+  if (!cfg || !out) return fail(SDTFE_EINVAL, "null argument");
+  *out = nullptr;
+  if (cfg->hbm_budget_bytes <= 0 || cfg->max_active <= 0 || cfg->tile_budget <= 0)
+    return fail(SDTFE_EINVAL, "bad server configuration");
+  if (cfg->precision != SDTFE_PREC_FP64 && cfg->precision != SDTFE_PREC_MIXED)
+    return fail(SDTFE_EINVAL, "unknown precision");
+  sdtfe_server *sv = new sdtfe_server();
+  sv->cfg = *cfg;
+  int rc = sdtfe_batch_create(&sv->batch);
+  if (rc) {
+    delete sv;
+    return rc;
+  }
+  sv->t0 = Clock::now();
+  sv->setup_thr = std::thread(setup_loop, sv);
+  sv->walk_thr = std::thread(walk_loop, sv);
+  *out = sv;
+  return 0;
+}
+
+int sdtfe_server_submit(sdtfe_server *sv, const double *particle_data, int n_particles,
+                        const int *tetra_data, int n_tetra, float p_mass,
+                        const sdtfe_map_params *p, double *rho, int64_t *request_id) {
+  if (!sv || !p || !rho || n_particles < 0 || n_tetra < 0 ||
+      (n_particles && !particle_data) || (n_tetra && !tetra_data))
+    return fail(SDTFE_EINVAL, "bad argument");
+  if (p->precision != sv->cfg.precision)
+    return fail(SDTFE_EINVAL, "request precision differs from the server's");
+  if (p->grid_dim <= 0 || p->grid_dim > 65536 || p->n_mc_samp < 0 || !(p->box_len > 0) ||
+      !(p->box_len_z > 0))
+    return fail(SDTFE_EINVAL, "bad map parameters");
+  auto r = std::make_shared<Request>();
+  r->pd.assign(particle_data, particle_data + 4 * (size_t)n_particles);
+  r->tets.assign(tetra_data, tetra_data + 8 * (size_t)n_tetra);
+  r->p_mass = p_mass;
+  r->params = *p;
+  r->out = rho;
+  r->footprint = estimate_footprint(n_particles, n_tetra, p->grid_dim);
+  r->t_submit = Clock::now();
+  std::lock_guard<std::mutex> g(sv->mu);
+  r->id = sv->next_id++;
+  sv->all[r->id] = r;
+  sv->queued.push_back(r);
+  if (request_id) *request_id = r->id;
+  sv->cv_setup.notify_all();
+  return 0;
+}
+
+int sdtfe_server_wait(sdtfe_server *sv, int64_t id, sdtfe_request_info *info) {
+  if (!sv) return fail(SDTFE_EINVAL, "null server");
+  std::shared_ptr<Request> r;
+  {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->all.find(id);
+    if (it == sv->all.end()) return fail(SDTFE_EINVAL, "unknown request id");
+    r = it->second;
+    sv->cv_done.wait(lk, [&] { return r->done; });
+    sv->all.erase(id);
+  }
+  if (info) {
+    info->status = r->status;
+    info->queue_ms = ms_between(r->t_submit, r->t_admit);
+    info->setup_ms = r->started ? ms_between(r->t_admit, r->t_start) : 0.0;
+    info->latency_ms = ms_between(r->t_submit, r->t_done);
+    info->submit_ms = ms_between(sv->t0, r->t_submit);
+    info->done_ms = ms_between(sv->t0, r->t_done);
+    info->footprint_bytes = r->footprint;
+    info->stats = r->stats;
+  }
+  return r->status;
+}
+
+int sdtfe_server_get_info(sdtfe_server *sv, sdtfe_server_info *info) {
+  if (!sv || !info) return fail(SDTFE_EINVAL, "null argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  info->queued = (int)sv->queued.size();
+  info->active = (int)sv->active.size() + sv->n_setup;
+  info->used_bytes = sv->used_bytes;
+  info->launches = sv->launches;
+  info->walk_ms = sv->walk_ms;
+  return 0;
+}
+
+void sdtfe_server_destroy(sdtfe_server *sv) {
+  if (!sv) return;
+  {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    // drain: wait until nothing is queued, resident or being set up
+    sv->cv_done.wait(lk, [&] {
+      return sv->queued.empty() && sv->active.empty() && sv->n_setup == 0;
+    });
+    sv->stop = true;
+    sv->cv_setup.notify_all();
+    sv->cv_walk.notify_all();
+  }
+  if (sv->setup_thr.joinable()) sv->setup_thr.join();
+  if (sv->walk_thr.joinable()) sv->walk_thr.join();
+  sdtfe_batch_destroy(sv->batch);
+  delete sv;
+}
+
+}  // extern "C"

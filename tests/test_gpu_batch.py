@@ -1,0 +1,67 @@
+"""Batched persistent launch: every item's map equals its standalone map
+(FP64: bit for bit, whatever the sample tiling), across uneven cutouts and
+repeated graph replays."""
+import sys
+
+import numpy as np
+import pytest
+
+from sdtfe_amd import autotune, capi, workloads
+from sdtfe_amd.tessellate import to_particle_data, triangulate
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cutouts():
+    specs = [(1500, 3, 64, 4), (6000, 4, 100, 7), (20000, 5, 48, 16), (3000, 6, 128, 5)]
+    out = []
+    for n, seed, G, S in specs:
+        pos = workloads.cutout(n, seed)
+        t = capi.Tessellation(to_particle_data(pos), triangulate(pos))
+        out.append((t, G, S, seed))
+    yield out
+    for t, *_ in out:
+        t.close()
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_FP64, capi.PREC_MIXED])
+@pytest.mark.parametrize("lpp", [1, 4])
+def test_batch_equals_standalone(cutouts, prec, lpp):
+    _say(f"[gpu] batch prec={prec} lanes/pixel={lpp}")
+    alone = [t.map(G, S, seed=seed, precision=prec)[0] for t, G, S, seed in cutouts]
+    bufs = [capi.DeviceBuffer(G * G * 8) for _, G, _, _ in cutouts]
+    with capi.Batch() as b:
+        for rep in range(2):  # second run replays the captured graph
+            for buf in bufs:
+                buf.zero()
+            items = [(t, t.params(G, S, seed=seed, precision=prec), buf)
+                     for (t, G, S, seed), buf in zip(cutouts, bufs)]
+            st, tot = b.run(items, lanes_per_pixel=[lpp] * len(items))
+            for (t, G, S, seed), buf, ref, s in zip(cutouts, bufs, alone, st):
+                got = buf.to_host()
+                if prec == capi.PREC_FP64:
+                    np.testing.assert_array_equal(got, ref)
+                else:
+                    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=0)
+                assert s["samples"] == G * G * S
+    for buf in bufs:
+        buf.close()
+    assert tot["t_walk_ms"] > 0
+
+
+def test_autotuner_picks_a_candidate(cutouts, tmp_path):
+    _say("[gpu] autotuner")
+    t, G, S, seed = cutouts[2]
+    tuner = autotune.Autotuner(str(tmp_path / "tune.json"), candidates=(1, 4), repeats=1)
+    with capi.Batch() as b:
+        lpp = tuner.lanes_for(b, t, t.params(G, S, seed=seed))
+    assert lpp in (1, 4)
+    tuner.save()
+    again = autotune.Autotuner(str(tmp_path / "tune.json"))
+    assert again.table == tuner.table
