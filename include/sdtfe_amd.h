@@ -131,6 +131,9 @@ int sdtfe_tess_get_info(const sdtfe_tess *t, sdtfe_tess_info *info);
 int sdtfe_tess_vertex_density(const sdtfe_tess *t, double *rho);
 int sdtfe_tess_gradients(const sdtfe_tess *t, double *nabla);
 int sdtfe_tess_triangles(const sdtfe_tess *t, int *tri_rows, int capacity);
+/* The FP64 walk records, 144 bytes per tet: neighbours (4 x int32), A, B,
+ * C, D coordinates (12 x float64), rho_A and the gradient (4 x float64). */
+int sdtfe_tess_walk_records(const sdtfe_tess *t, void *records, size_t bytes);
 
 /* Entry tet of each query point (-1 outside the hull's shadow). */
 int sdtfe_tess_locate(sdtfe_tess *t, int n, const double *x, const double *y, int *entry);

@@ -79,6 +79,7 @@ _SIGS = {
     "sdtfe_tess_vertex_density": (C.c_int, [_vp, _dp]),
     "sdtfe_tess_gradients": (C.c_int, [_vp, _dp]),
     "sdtfe_tess_triangles": (C.c_int, [_vp, _ip, C.c_int]),
+    "sdtfe_tess_walk_records": (C.c_int, [_vp, _vp, C.c_size_t]),
     "sdtfe_tess_locate": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip]),
     "sdtfe_tess_trace": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _vp, _u64p, _u32p,
                                    C.c_double, C.c_double, C.c_double, C.c_uint64, _dp, _ip,
@@ -237,6 +238,13 @@ class Tessellation:
         out = np.zeros((max(f, 1), 8), dtype=np.int32)
         check(lib().sdtfe_tess_triangles(self.h, out, f))
         return out[:f]
+
+    REC64 = np.dtype([("nb", "<i4", 4), ("v", "<f8", 12), ("rho_a", "<f8"), ("g", "<f8", 3)])
+
+    def walk_records(self) -> np.ndarray:
+        out = np.zeros(self.nt, dtype=self.REC64)
+        check(lib().sdtfe_tess_walk_records(self.h, out.ctypes.data, out.nbytes))
+        return out
 
     def locate(self, x, y) -> np.ndarray:
         x, y = _c64(x), _c64(y)

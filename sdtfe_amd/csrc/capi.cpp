@@ -280,6 +280,16 @@ int sdtfe_tess_triangles(const sdtfe_tess *t, int *rows, int capacity) {
   return 0;
 }
 
+int sdtfe_tess_walk_records(const sdtfe_tess *t, void *records, size_t bytes) {
+  if (!t || bytes < sizeof(Rec64) * (size_t)t->nt || (!records && t->nt))
+    return fail(SDTFE_EINVAL, "bad argument");
+  if (t->nt == 0) return 0;
+  SDTFE_HIP(hipMemcpyAsync(records, t->rec64, sizeof(Rec64) * (size_t)t->nt,
+                           hipMemcpyDeviceToHost, t->stream));
+  SDTFE_HIP(hipStreamSynchronize(t->stream));
+  return 0;
+}
+
 int sdtfe_tess_locate(sdtfe_tess *t, int n, const double *x, const double *y, int *entry) {
   if (!t || n < 0 || (n && (!x || !y || !entry))) return fail(SDTFE_EINVAL, "bad argument");
   if (n == 0) return 0;

@@ -35,9 +35,8 @@ SDTFE_HD inline void load_tet(const Rec64 *__restrict__ rec, int t, Tet64 &r) {
   r.rho_a = q6.x; r.gx = q6.y; r.gy = q7.x; r.gz = q7.y;
 }
 
-SDTFE_HD inline int nb_of(const Tet64 &r, int f) {
-  return f == 0 ? r.nb0 : (f == 1 ? r.nb1 : (f == 2 ? r.nb2 : r.nb3));
-}
+SDTFE_HD inline int pick4i(int f, int a, int b, int c, int d);
+SDTFE_HD inline int nb_of(const Tet64 &r, int f) { return pick4i(f, r.nb0, r.nb1, r.nb2, r.nb3); }
 
 // Plücker product of the z-directed ray with the edge (X, Y), formed as
 // the reference forms it: (X - Y, X.x*Y.y - X.y*Y.x) against the ray's
@@ -55,26 +54,56 @@ struct Hit {
   bool ok, special;
 };
 
-// Crossing height of face f: barycentric weights from the face's three
-// signed edge products, point = (1 - w1 - w2) p0 + w1 p1 + w2 p2
-// (src/dtfe.c:67-74, 112-119, 151-158, 184-191).
-SDTFE_HD inline double face_z(int f, double uCB, double uAC, double uBA, double uDC,
-                                double uBD, double uDA, const Tet64 &r) {
-  double t0, t1, t2, w1n, w2n, z0, z1, z2;
-  if (f == 0) {  // BDC
-    t0 = -uCB; t1 = uDC; t2 = -uBD; w1n = t2; w2n = t1; z0 = r.dz; z1 = r.cz; z2 = r.bz;
-  } else if (f == 1) {  // ACD
-    t0 = uAC; t1 = -uDC; t2 = -uDA; w1n = t0; w2n = t1; z0 = r.cz; z1 = r.dz; z2 = r.az;
-  } else if (f == 2) {  // ADB
-    t0 = uDA; t1 = uBD; t2 = -uBA; w1n = t1; w2n = t2; z0 = r.bz; z1 = r.az; z2 = r.dz;
-  } else {  // ABC
-    t0 = uBA; t1 = uCB; t2 = -uAC; w1n = t2; w2n = t0; z0 = r.az; z1 = r.bz; z2 = r.cz;
-  }
+// Per-lane choice among four values without control flow.  hipcc (ROCm
+// 7.2, gfx950) miscompiles an if/else (or ?:) chain that picks
+// floating-point operands by a lane-varying face index: lanes on the last arm
+// lose that arm's assignments.  Every such choice here computes all
+// candidates in straight-line code and picks with bit masks.
+SDTFE_HD inline double pick4(int f, double a, double b, double c, double d) {
+  const uint64_t m0 = 0ull - (uint64_t)(f == 0), m1 = 0ull - (uint64_t)(f == 1);
+  const uint64_t m2 = 0ull - (uint64_t)(f == 2), m3 = 0ull - (uint64_t)(f == 3);
+  const uint64_t v = (__builtin_bit_cast(uint64_t, a) & m0) | (__builtin_bit_cast(uint64_t, b) & m1) |
+                     (__builtin_bit_cast(uint64_t, c) & m2) | (__builtin_bit_cast(uint64_t, d) & m3);
+  return __builtin_bit_cast(double, v);
+}
+
+SDTFE_HD inline float pick4f(int f, float a, float b, float c, float d) {
+  const uint32_t m0 = 0u - (uint32_t)(f == 0), m1 = 0u - (uint32_t)(f == 1);
+  const uint32_t m2 = 0u - (uint32_t)(f == 2), m3 = 0u - (uint32_t)(f == 3);
+  const uint32_t v = (__builtin_bit_cast(uint32_t, a) & m0) | (__builtin_bit_cast(uint32_t, b) & m1) |
+                     (__builtin_bit_cast(uint32_t, c) & m2) | (__builtin_bit_cast(uint32_t, d) & m3);
+  return __builtin_bit_cast(float, v);
+}
+
+SDTFE_HD inline int pick4i(int f, int a, int b, int c, int d) {
+  const int m0 = -(int)(f == 0), m1 = -(int)(f == 1), m2 = -(int)(f == 2), m3 = -(int)(f == 3);
+  return (a & m0) | (b & m1) | (c & m2) | (d & m3);
+}
+
+// Crossing heights of the four faces: barycentric weights from each face's
+// three signed edge products, point = (1 - w1 - w2) p0 + w1 p1 + w2 p2
+// (src/dtfe.c:67-74, 112-119, 151-158, 184-191), straight-line.
+struct FaceHeights {
+  double z[4];
+};
+
+SDTFE_HD inline double face_height(double t0, double t1, double t2, double w1n, double w2n,
+                                   double z0, double z1, double z2) {
   double den = t0 + t1;
   den = den + t2;
   const double inv = 1.0 / den;
   const double w1 = w1n * inv, w2 = w2n * inv;
   return (1 - w1 - w2) * z0 + w1 * z1 + w2 * z2;
+}
+
+SDTFE_HD inline FaceHeights face_heights(double uCB, double uAC, double uBA, double uDC,
+                                         double uBD, double uDA, const Tet64 &r) {
+  FaceHeights h;
+  h.z[0] = face_height(-uCB, uDC, -uBD, -uBD, uDC, r.dz, r.cz, r.bz);   // BDC
+  h.z[1] = face_height(uAC, -uDC, -uDA, uAC, -uDC, r.cz, r.dz, r.az);   // ACD
+  h.z[2] = face_height(uDA, uBD, -uBA, uBD, -uBA, r.bz, r.az, r.dz);    // ADB
+  h.z[3] = face_height(uBA, uCB, -uAC, -uAC, uBA, r.az, r.bz, r.cz);    // ABC
+  return h;
 }
 
 // rayTetraPlucker_full (src/dtfe.c:28-215): faces are tested in the order
@@ -112,11 +141,9 @@ SDTFE_HD inline Hit plucker64(const Tet64 &r, double lz, double rx, double ry) {
     found += (en || le) ? 1 : 0;
   }
   h.ok = found == 2;
-  h.z_in = h.z_out = 0.0;
-  if (h.ok) {
-    h.z_in = face_z(h.enter, uCB, uAC, uBA, uDC, uBD, uDA, r);
-    h.z_out = face_z(h.leave, uCB, uAC, uBA, uDC, uBD, uDA, r);
-  }
+  const FaceHeights zf = face_heights(uCB, uAC, uBA, uDC, uBD, uDA, r);
+  h.z_in = pick4(h.enter, zf.z[0], zf.z[1], zf.z[2], zf.z[3]);
+  h.z_out = pick4(h.leave, zf.z[0], zf.z[1], zf.z[2], zf.z[3]);
   return h;
 }
 
@@ -129,9 +156,9 @@ SDTFE_HD inline bool resolve_special(const Tet64 &r, double lz, double step, uin
                                        LaneCounters &cnt) {
   int tries = 0;
   while (h.special) {
-    const uint32_t k = draw31(key, draw++) & 3u;
-    const double vx = k == 0 ? r.ax : (k == 1 ? r.bx : (k == 2 ? r.cx : r.dx));
-    const double vy = k == 0 ? r.ay : (k == 1 ? r.by : (k == 2 ? r.cy : r.dy));
+    const int k = (int)(draw31(key, draw++) & 3u);
+    const double vx = pick4(k, r.ax, r.bx, r.cx, r.dx);
+    const double vy = pick4(k, r.ay, r.by, r.cy, r.dy);
     double d0 = vx - ox, d1 = vy - oy;
     const double mag = sqrt(d0 * d0 + d1 * d1);
     if (mag > step) {

@@ -55,9 +55,7 @@ SDTFE_HD inline void load_tet32(const Rec32 *__restrict__ rec, int t, Tet32 &r) 
   r.dx = q2.y; r.dy = q2.z; r.dz = q2.w;
 }
 
-SDTFE_HD inline int nb32(const Tet32 &r, int f) {
-  return f == 0 ? r.nb0 : (f == 1 ? r.nb1 : (f == 2 ? r.nb2 : r.nb3));
-}
+SDTFE_HD inline int nb32(const Tet32 &r, int f) { return pick4i(f, r.nb0, r.nb1, r.nb2, r.nb3); }
 
 // Per-map constants of the filter (host-computed, see filter_consts()).
 struct Filter {
@@ -87,40 +85,46 @@ SDTFE_HD inline Side side32(float x1, float y1, float l1a, float x2, float y2, f
   return s;
 }
 
-// Crossing point of face f from FP32 side values: height and density there,
-// both the barycentric mix of the face's three vertices with the weights the
-// FP64 walk uses for the height (face_z), written relative to the face's
-// first vertex.  rho holds the vertex densities (A, B, C, D).
+// Crossing points of the four faces from FP32 side values: height and
+// density there, both the barycentric mix of the face's three vertices with
+// the weights the FP64 walk uses for the height, written relative to the
+// face's first vertex.  rho holds the vertex densities (A, B, C, D).
+// Straight-line for all faces; callers pick one (see pick4).
 struct Cross {
   float z, rho;
 };
 
-SDTFE_HD inline Cross face_cross32(int f, float cCB, float cAC, float cBA, float cDC, float cBD,
-                                   float cDA, const Tet32 &r, const float4 &rho) {
-  float t0, t1, t2, w1n, w2n, z0, z1, z2, q0, q1, q2;
-  if (f == 0) {  // BDC: D + w1 (C - D) + w2 (B - D)
-    t0 = -cCB; t1 = cDC; t2 = -cBD; w1n = t2; w2n = t1;
-    z0 = r.dz; z1 = r.cz; z2 = r.bz; q0 = rho.w; q1 = rho.z; q2 = rho.y;
-  } else if (f == 1) {  // ACD: C + w1 (D - C) + w2 (A - C)
-    t0 = cAC; t1 = -cDC; t2 = -cDA; w1n = t0; w2n = t1;
-    z0 = r.cz; z1 = r.dz; z2 = r.az; q0 = rho.z; q1 = rho.w; q2 = rho.x;
-  } else if (f == 2) {  // ADB: B + w1 (A - B) + w2 (D - B)
-    t0 = cDA; t1 = cBD; t2 = -cBA; w1n = t1; w2n = t2;
-    z0 = r.bz; z1 = r.az; z2 = r.dz; q0 = rho.y; q1 = rho.x; q2 = rho.w;
-  } else {  // ABC: A + w1 (B - A) + w2 (C - A)
-    t0 = cBA; t1 = cCB; t2 = -cAC; w1n = t2; w2n = t0;
-    z0 = r.az; z1 = r.bz; z2 = r.cz; q0 = rho.x; q1 = rho.y; q2 = rho.z;
-  }
+struct Crosses {
+  float z[4], rho[4];
+};
+
+SDTFE_HD inline void face_point32(float t0, float t1, float t2, float w1n, float w2n, float z0,
+                                  float z1, float z2, float q0, float q1, float q2, float &z,
+                                  float &q) {
   const float inv = 1.0f / (t0 + t1 + t2);
   const float w1 = w1n * inv, w2 = w2n * inv;
-  Cross c;
-  c.z = fmaf(w1, z1 - z0, fmaf(w2, z2 - z0, z0));
-  c.rho = fmaf(w1, q1 - q0, fmaf(w2, q2 - q0, q0));
+  z = fmaf(w1, z1 - z0, fmaf(w2, z2 - z0, z0));
+  q = fmaf(w1, q1 - q0, fmaf(w2, q2 - q0, q0));
+}
+
+SDTFE_HD inline Crosses face_crosses32(float cCB, float cAC, float cBA, float cDC, float cBD,
+                                       float cDA, const Tet32 &r, const float4 &rho) {
+  Crosses c;
+  // BDC: D + w1 (C - D) + w2 (B - D); ACD: C, D, A; ADB: B, A, D; ABC: A, B, C
+  face_point32(-cCB, cDC, -cBD, -cBD, cDC, r.dz, r.cz, r.bz, rho.w, rho.z, rho.y, c.z[0], c.rho[0]);
+  face_point32(cAC, -cDC, -cDA, cAC, -cDC, r.cz, r.dz, r.az, rho.z, rho.w, rho.x, c.z[1], c.rho[1]);
+  face_point32(cDA, cBD, -cBA, cBD, -cBA, r.bz, r.az, r.dz, rho.y, rho.x, rho.w, c.z[2], c.rho[2]);
+  face_point32(cBA, cCB, -cAC, -cAC, cBA, r.az, r.bz, r.cz, rho.x, rho.y, rho.z, c.z[3], c.rho[3]);
   return c;
 }
 
-// Same crossing from the FP64 record: side products with the reference
-// expression, weights and height in FP64 (the density mixes the FP32 vertex
+SDTFE_HD inline Cross pick_cross(int f, const Crosses &c) {
+  return Cross{pick4f(f, c.z[0], c.z[1], c.z[2], c.z[3]),
+               pick4f(f, c.rho[0], c.rho[1], c.rho[2], c.rho[3])};
+}
+
+// The same crossing from the FP64 record (side products with the reference
+// expression, weights and height in FP64; the density mixes the FP32 vertex
 // densities).  Height returned relative to org_z.
 SDTFE_HD inline Cross face_cross64(int f, const Tet64 &q, double lz, double ox, double oy,
                                    const float4 &rho, double org_z) {
@@ -131,42 +135,36 @@ SDTFE_HD inline Cross face_cross64(int f, const Tet64 &q, double lz, double ox, 
   const double uDC = edge_u(p2, p3, p4, q.cx, q.cy, q.dx, q.dy);
   const double uBD = edge_u(p2, p3, p4, q.bx, q.by, q.dx, q.dy);
   const double uDA = edge_u(p2, p3, p4, q.dx, q.dy, q.ax, q.ay);
-  double t0, t1, t2, w1n, w2n, z0, z1, z2, r0, r1, r2;
-  if (f == 0) {
-    t0 = -uCB; t1 = uDC; t2 = -uBD; w1n = t2; w2n = t1;
-    z0 = q.dz; z1 = q.cz; z2 = q.bz; r0 = rho.w; r1 = rho.z; r2 = rho.y;
-  } else if (f == 1) {
-    t0 = uAC; t1 = -uDC; t2 = -uDA; w1n = t0; w2n = t1;
-    z0 = q.cz; z1 = q.dz; z2 = q.az; r0 = rho.z; r1 = rho.w; r2 = rho.x;
-  } else if (f == 2) {
-    t0 = uDA; t1 = uBD; t2 = -uBA; w1n = t1; w2n = t2;
-    z0 = q.bz; z1 = q.az; z2 = q.dz; r0 = rho.y; r1 = rho.x; r2 = rho.w;
-  } else {
-    t0 = uBA; t1 = uCB; t2 = -uAC; w1n = t2; w2n = t0;
-    z0 = q.az; z1 = q.bz; z2 = q.cz; r0 = rho.x; r1 = rho.y; r2 = rho.z;
+  double zz[4], rr[4];
+  const double ra = rho.x, rb = rho.y, rc = rho.z, rd = rho.w;
+  const double t[4][5] = {{-uCB, uDC, -uBD, -uBD, uDC},
+                          {uAC, -uDC, -uDA, uAC, -uDC},
+                          {uDA, uBD, -uBA, uBD, -uBA},
+                          {uBA, uCB, -uAC, -uAC, uBA}};
+  const double zv[4][3] = {{q.dz, q.cz, q.bz}, {q.cz, q.dz, q.az}, {q.bz, q.az, q.dz},
+                           {q.az, q.bz, q.cz}};
+  const double rv[4][3] = {{rd, rc, rb}, {rc, rd, ra}, {rb, ra, rd}, {ra, rb, rc}};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double inv = 1.0 / (t[k][0] + t[k][1] + t[k][2]);
+    const double w1 = t[k][3] * inv, w2 = t[k][4] * inv;
+    zz[k] = zv[k][0] + w1 * (zv[k][1] - zv[k][0]) + w2 * (zv[k][2] - zv[k][0]);
+    rr[k] = rv[k][0] + w1 * (rv[k][1] - rv[k][0]) + w2 * (rv[k][2] - rv[k][0]);
   }
-  const double inv = 1.0 / (t0 + t1 + t2);
-  const double w1 = w1n * inv, w2 = w2n * inv;
   Cross c;
-  c.z = (float)(z0 + w1 * (z1 - z0) + w2 * (z2 - z0) - org_z);
-  c.rho = (float)(r0 + w1 * (r1 - r0) + w2 * (r2 - r0));
+  c.z = (float)(pick4(f, zz[0], zz[1], zz[2], zz[3]) - org_z);
+  c.rho = (float)pick4(f, rr[0], rr[1], rr[2], rr[3]);
   return c;
 }
 
 // Relative uncertainty of face f's FP32 barycentric weights.
 SDTFE_HD inline float face_cond(int f, const Side &CB, const Side &AC, const Side &BA,
                                 const Side &DC, const Side &BD, const Side &DA) {
-  float den, b;
-  if (f == 0) {
-    den = -CB.c + DC.c - BD.c; b = CB.b + DC.b + BD.b;
-  } else if (f == 1) {
-    den = AC.c - DC.c - DA.c; b = AC.b + DC.b + DA.b;
-  } else if (f == 2) {
-    den = DA.c + BD.c - BA.c; b = DA.b + BD.b + BA.b;
-  } else {
-    den = BA.c + CB.c - AC.c; b = BA.b + CB.b + AC.b;
-  }
-  return b / fabsf(den);
+  const float c0 = (CB.b + DC.b + BD.b) / fabsf(-CB.c + DC.c - BD.c);
+  const float c1 = (AC.b + DC.b + DA.b) / fabsf(AC.c - DC.c - DA.c);
+  const float c2 = (DA.b + BD.b + BA.b) / fabsf(DA.c + BD.c - BA.c);
+  const float c3 = (BA.b + CB.b + AC.b) / fabsf(BA.c + CB.c - AC.c);
+  return pick4f(f, c0, c1, c2, c3);
 }
 
 // Face choice from the six side signs (src/dtfe.c:64-213, as plucker64).
@@ -301,16 +299,14 @@ SDTFE_HD inline double walk_mixed(const Rec32 *__restrict__ rec32, const Field32
     const bool in64 = !have_carry;
     const bool out64 = nb32(r, leave) == -1 ||
                        face_cond(leave, CB, AC, BA, DC, BD, DA) > kCondMax;
-    Cross in = carry, out;
+    const Crosses cr = face_crosses32(CB.c, AC.c, BA.c, DC.c, BD.c, DA.c, r, rho4);
+    Cross in = carry, out = pick_cross(leave, cr);
     if (in64 | out64) {
       Tet64 q;
       load_tet(rec64, cur, q);
       if (in64) in = face_cross64(enter, q, g.lz, ox, oy, rho4, g.org_z);
-      out = out64 ? face_cross64(leave, q, g.lz, ox, oy, rho4, g.org_z)
-                  : face_cross32(leave, CB.c, AC.c, BA.c, DC.c, BD.c, DA.c, r, rho4);
+      if (out64) out = face_cross64(leave, q, g.lz, ox, oy, rho4, g.org_z);
       cnt.fp64_redo++;
-    } else {
-      out = face_cross32(leave, CB.c, AC.c, BA.c, DC.c, BD.c, DA.c, r, rho4);
     }
     carry = out;
     have_carry = true;
