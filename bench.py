@@ -82,8 +82,11 @@ def time_form(tess, cfg, precision, steps, warmup, seed):
         st_last = st
     tess.sync()
     wall = time.perf_counter() - t0
-    # the accumulated map over `steps` identical maps, divided back
-    acc = buf.to_host() / steps
+    # one more map, untimed, into a clean buffer: the map compared across forms
+    buf.zero()
+    tess.map_device(p, buf, stats=False)
+    tess.sync()
+    acc = buf.to_host()
     buf.close()
     return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, walk_ms=float(np.mean(walk_ms)),
                 tet_steps=int(np.mean(steps_per_map)), stats=st_last, map=acc)
@@ -128,6 +131,33 @@ def cpu_baseline(pos, tetra, cfg, seed, target_s, gpu_map):
                 gpu_vs_cpu_max_rel_err=float(err.max()) if err.size else 0.0)
 
 
+def tessellation(name, pos, cache):
+    """Host Qhull tessellation, optionally cached in an .npz (e.g. under /tmp
+    between profiling passes of one session)."""
+    from sdtfe_amd.tessellate import triangulate
+    if cache and os.path.exists(cache):
+        with np.load(cache) as z:
+            if z["n"] == pos.shape[0] and str(z["name"]) == name:
+                return z["tetra"]
+    tetra = triangulate(pos)
+    if cache:
+        np.savez(cache, tetra=tetra, n=pos.shape[0], name=name)
+    return tetra
+
+
+def copy_calibration(gib: float = 1.0, reps: int = 3):
+    """A plain device copy of known size (torch), for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,7 +168,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tess-cache", default=None, help="npz file caching the host tessellation")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the path shards by "
               "whole maps or 4-row bands, which this build does not run across GPUs.",
@@ -146,13 +182,13 @@ def main():
         sys.exit(2)
 
     from sdtfe_amd import capi
-    from sdtfe_amd.tessellate import to_particle_data, triangulate
+    from sdtfe_amd.tessellate import to_particle_data
     cfg = CONFIGS[a.config]
     G, S = cfg["grid"], cfg["samples"]
     log(f"[bench] generating {a.config} particles")
     pos = particles(a.config)
     t0 = time.perf_counter()
-    tetra = triangulate(pos)
+    tetra = tessellation(a.config, pos, a.tess_cache)
     log(f"[bench] tessellated {pos.shape[0]} particles -> {tetra.shape[0]} tets in "
         f"{time.perf_counter() - t0:.1f} s")
     assert capi.device_count() >= 1, "no HIP device"

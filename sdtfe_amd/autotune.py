@@ -58,4 +58,5 @@ class Autotuner:
         finally:
             buf.close()
         pick = min(timings, key=timings.get)
-        return {"lanes_per_pixel": pick, "walk_ms": timings}
+        # JSON keys are strings
+        return {"lanes_per_pixel": pick, "walk_ms": {str(k): v for k, v in timings.items()}}

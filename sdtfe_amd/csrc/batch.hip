@@ -33,7 +33,6 @@ constexpr int kLdsBytes = 32 * 1024;
 struct BatchItem {
   const Rec64 *rec64;
   const Rec32 *rec32;
-  const Field32 *fld;
   EntryGrid grid;
   MapGeom geom;
   MixedConsts mc;
@@ -75,8 +74,7 @@ template <bool kMixed>
 __device__ inline double walk_one(const BatchItem &it, int entry, double x, double y,
                                   uint64_t key, int &crossed, LaneCounters &cnt) {
   if (kMixed)
-    return walk_mixed(it.rec32, it.fld, it.rec64, entry, x, y, it.geom, it.mc.flt, it.mc.zlo,
-                      it.mc.zhi, it.mc.zband, key, crossed, cnt);
+    return walk_mixed(it.rec32, it.rec64, entry, x, y, it.geom, it.mc.flt, key, crossed, cnt);
   return walk64(it.rec64, entry, x, y, it.geom, key, crossed, cnt);
 }
 
@@ -298,7 +296,6 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
     std::memset(&it, 0, sizeof it);
     it.rec64 = tess[i]->rec64;
     it.rec32 = tess[i]->rec32;
-    it.fld = tess[i]->fld32;
     it.grid = tess[i]->grid;
     it.geom = make_geom_pub(tess[i], p);
     it.mc = mixed_consts(tess[i], it.geom);

@@ -139,9 +139,9 @@ struct DevArray {
 
 void free_tess(sdtfe_tess *t) {
   if (!t) return;
-  void *bufs[] = {t->pos4,     t->tets,    t->vol,      t->nabla,    t->rec64,
-                  t->rec32,    t->fld32,   t->tri_xy,   t->tri_tet,  t->tri_abc,
-                  t->cell_off, t->cell_tri, t->stats,   t->map_buf,  t->navg_buf};
+  void *bufs[] = {t->pos4,    t->tets,     t->vol,     t->nabla,   t->rec64,
+                  t->rec32,   t->tri_xy,   t->tri_tet, t->tri_abc, t->cell_off,
+                  t->cell_tri, t->stats,   t->map_buf, t->navg_buf};
   if (t->stream) (void)hipStreamSynchronize(t->stream);
   for (void *b : bufs)
     if (b) (void)hipFree(b);

@@ -210,8 +210,7 @@ __device__ inline double det3(double a, double b, double c, double d, double e, 
 }
 
 __global__ void gradient_record_kernel(const double4 *pos, const int *tets, int nt,
-                                       double *nabla, Rec64 *rec, Rec32 *rec32,
-                                       Field32 *fld, double ox, double oy, double oz) {
+                                       double *nabla, Rec64 *rec, Rec32 *rec32) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   const int4 v = tet_verts(tets, t), nb = tet_nbrs(tets, t);
@@ -239,15 +238,13 @@ __global__ void gradient_record_kernel(const double4 *pos, const int *tets, int 
 
   Rec32 q;
   q.nb[0] = nb.x; q.nb[1] = nb.y; q.nb[2] = nb.z; q.nb[3] = nb.w;
-  const double4 *p4[4] = {&a, &b, &c, &d};
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    q.v[3 * k + 0] = (float)(p4[k]->x - ox);
-    q.v[3 * k + 1] = (float)(p4[k]->y - oy);
-    q.v[3 * k + 2] = (float)(p4[k]->z - oz);
-  }
+  q.a[0] = a.x; q.a[1] = a.y; q.a[2] = a.z;
+  q.e[0] = (float)(b.x - a.x); q.e[1] = (float)(b.y - a.y); q.e[2] = (float)(b.z - a.z);
+  q.e[3] = (float)(c.x - a.x); q.e[4] = (float)(c.y - a.y); q.e[5] = (float)(c.z - a.z);
+  q.e[6] = (float)(d.x - a.x); q.e[7] = (float)(d.y - a.y); q.e[8] = (float)(d.z - a.z);
+  q.rho[0] = (float)a.w; q.rho[1] = (float)b.w; q.rho[2] = (float)c.w; q.rho[3] = (float)d.w;
+  q.pad = 0.f;
   rec32[t] = q;
-  fld[t] = Field32{(float)a.w, (float)b.w, (float)c.w, (float)d.w};
 }
 
 // ============================================================ entry grid
@@ -436,7 +433,7 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra) {
     vertex_density_kernel<<<blocks_for(n, 256), 256, 0, s>>>(
         reinterpret_cast<double4 *>(t->pos4), n, off, inc, t->vol, m4);
 
-  // origin of the FP32 records: centre of the particles' bounding box
+  // centre and half-extent of the particles' bounding box
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int i = 0; i < n; ++i)
     for (int k = 0; k < 3; ++k) {
@@ -452,11 +449,9 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra) {
   SDTFE_RC(dev_alloc(t, &t->nabla, 3 * (size_t)nt));
   SDTFE_RC(dev_alloc(t, &t->rec64, (size_t)nt));
   SDTFE_RC(dev_alloc(t, &t->rec32, (size_t)nt));
-  SDTFE_RC(dev_alloc(t, &t->fld32, (size_t)nt));
   if (nt > 0)
     gradient_record_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
-        reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->nabla, t->rec64, t->rec32,
-        t->fld32, t->org[0], t->org[1], t->org[2]);
+        reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->nabla, t->rec64, t->rec32);
   SDTFE_HIP(hipGetLastError());
   SDTFE_HIP(hipStreamSynchronize(s));
   return 0;

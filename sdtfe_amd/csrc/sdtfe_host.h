@@ -50,9 +50,8 @@ struct sdtfe_tess {
   double *nabla = nullptr;    // T x 3
   sdtfe::Rec64 *rec64 = nullptr;
   sdtfe::Rec32 *rec32 = nullptr;
-  sdtfe::Field32 *fld32 = nullptr;
-  double org[3] = {0, 0, 0};  // origin of the FP32 records
-  double span = 0;            // max |coordinate - origin|
+  double org[3] = {0, 0, 0};  // centre of the particles' bounding box
+  double span = 0;            // its half-extent (max over axes)
   // entry structure
   double *tri_xy = nullptr;
   int *tri_tet = nullptr;

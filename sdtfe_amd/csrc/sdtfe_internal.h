@@ -29,18 +29,18 @@ struct alignas(16) Rec64 {
 };
 static_assert(sizeof(Rec64) == 144, "Rec64 layout");
 
-// Mixed-precision walk: 64 B geometry record (coordinates relative to the
-// tessellation's origin, FP32) and a 16 B record of the four vertex
-// densities (FP32).
+// Mixed-precision walk record, 96 B: neighbours, vertex A in FP64 and the
+// edges B - A, C - A, D - A plus the four vertex densities in FP32.  Offsets
+// from a ray are built from A in FP64, so FP32 rounding stays a fraction of
+// the local tet size.
 struct alignas(16) Rec32 {
   int nb[4];
-  float v[12];
+  double a[3];
+  float e[9];
+  float rho[4];
+  float pad;
 };
-static_assert(sizeof(Rec32) == 64, "Rec32 layout");
-
-struct alignas(16) Field32 {
-  float rho_a, rho_b, rho_c, rho_d;
-};
+static_assert(sizeof(Rec32) == 96, "Rec32 layout");
 
 // Entry lookup: forward-hull triangles binned on a uniform 2D grid.
 struct EntryGrid {

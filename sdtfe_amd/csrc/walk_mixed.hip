@@ -5,7 +5,6 @@
 namespace sdtfe {
 
 __global__ __launch_bounds__(256) void map_mixed_kernel(const Rec32 *__restrict__ rec32,
-                                                       const Field32 *__restrict__ fld,
                                                        const Rec64 *__restrict__ rec64,
                                                        EntryGrid eg, MapGeom g, MixedConsts mc,
                                                        double *out, int *navg, int accumulate,
@@ -29,8 +28,7 @@ __global__ __launch_bounds__(256) void map_mixed_kernel(const Rec32 *__restrict_
         continue;
       }
       int crossed;
-      const double v = walk_mixed(rec32, fld, rec64, entry, xt, yt, g, mc.flt, mc.zlo, mc.zhi,
-                                  mc.zband, key, crossed, cnt);
+      const double v = walk_mixed(rec32, rec64, entry, xt, yt, g, mc.flt, key, crossed, cnt);
       sum += v;
       cnt.tet_steps += (unsigned)crossed;
       if (crossed == 0) {
@@ -49,7 +47,6 @@ __global__ __launch_bounds__(256) void map_mixed_kernel(const Rec32 *__restrict_
 }
 
 __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
-                                   const Field32 *__restrict__ fld,
                                    const Rec64 *__restrict__ rec64, EntryGrid eg, MapGeom g,
                                    MixedConsts mc, int n, const double *x, const double *y,
                                    const int *entry_in, const uint64_t *pixel,
@@ -61,8 +58,7 @@ __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
     const uint64_t key = sample_key(g.seed, pixel[i], sample[i]);
     const int entry = entry_in ? entry_in[i] : locate_entry(eg, x[i], y[i]);
     int crossed;
-    value[i] = walk_mixed(rec32, fld, rec64, entry, x[i], y[i], g, mc.flt, mc.zlo, mc.zhi,
-                          mc.zband, key, crossed, cnt);
+    value[i] = walk_mixed(rec32, rec64, entry, x[i], y[i], g, mc.flt, key, crossed, cnt);
     crossed_out[i] = crossed;
     cnt.tet_steps += (unsigned)crossed;
   }
@@ -73,7 +69,7 @@ extern const bool kHaveMixed = true;
 
 void launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
   const int nb = (g.grid + 15) / 16;
-  map_mixed_kernel<<<nb * nb, 256, 0, t->stream>>>(t->rec32, t->fld32, t->rec64, t->grid, g,
+  map_mixed_kernel<<<nb * nb, 256, 0, t->stream>>>(t->rec32, t->rec64, t->grid, g,
                                                    mixed_consts(t, g), out, navg, accumulate,
                                                    t->stats);
 }
@@ -82,7 +78,7 @@ void launch_trace_mixed(sdtfe_tess *t, const MapGeom &g, int n, const double *x,
                         const double *y, const int *entry, const uint64_t *pixel,
                         const uint32_t *sample, double *value, int *crossed) {
   trace_mixed_kernel<<<(n + 255) / 256, 256, 0, t->stream>>>(
-      t->rec32, t->fld32, t->rec64, t->grid, g, mixed_consts(t, g), n, x, y, entry, pixel,
+      t->rec32, t->rec64, t->grid, g, mixed_consts(t, g), n, x, y, entry, pixel,
       sample, value, crossed, t->stats);
 }
 

@@ -1,32 +1,35 @@
 // walk_mixed_core.h -- the mixed-precision ray walk (gfx950).
 //
-// Same tets, same faces, same perturbations as the FP64 walk; faster
-// arithmetic and a smaller record:
+// Same tets, same faces, same perturbations as the FP64 walk; less FP64
+// arithmetic and a smaller record (sdtfe::Rec32, 96 B: neighbours, vertex A
+// in FP64, B - A, C - A, D - A and the four vertex densities in FP32):
 //
+//  * Local coordinates.  The ray's offset from A is formed in FP64 and
+//    rounded to FP32; the other vertices' offsets add the stored edges.  The
+//    FP32 rounding error is therefore a fraction of the local tet size, not
+//    of the box, so tiny tets in halo cores lose nothing.
 //  * Side tests in FP32 with a filter.  The Plücker product of the ray with
-//    edge (X, Y) is lz * orient2d(p; X, Y) (src/dtfe.c:24-26 expanded), so it
-//    is formed in FP32 as (X - p) x (Y - p) from coordinates stored relative
-//    to the tessellation's origin.  An a-priori bound covers the rounding of
-//    the stored coordinates, of the differences and of the products; the
-//    FP32 sign is taken only when |c| exceeds that bound plus the reference's
-//    zero band (ZERO = 1e4 DBL_EPSILON, src/constants.h:6, scaled by 1/lz)
-//    plus the FP64 expression's own error.  Otherwise the product is redone
-//    in FP64 with the reference expression and Sign().  Every side sign is
-//    therefore the FP64 walk's sign, so the walk visits the same tets.
+//    edge (X, Y) is lz * orient2d(p; X, Y) (src/dtfe.c:24-26 expanded) and is
+//    formed as (X - p) x (Y - p).  An a-priori bound covers the rounding of
+//    the offsets and of the products; the FP32 sign is taken only when |c|
+//    exceeds that bound plus the reference's zero band (ZERO = 1e4
+//    DBL_EPSILON, src/constants.h:6, over lz) plus the FP64 expression's own
+//    error.  Otherwise the product is redone in FP64 with the reference
+//    expression and Sign().  Every side sign is therefore the FP64 walk's
+//    sign, so the walk visits the same tets.
 //  * A tet whose face choice is special (a zero side) or fails is handed to
 //    the FP64 core, which perturbs the ray exactly as the FP64 walk does.
-//  * The crossing height of each face is computed once (leave face, FP32) and
-//    carried as the next tet's entry height; errors in a crossing height then
-//    cancel to first order because the DTFE field is continuous across the
-//    face.  Midpoints near the slab boundary are decided in FP64.
-//  * The density at each crossing is the barycentric mix of the face's
-//    vertex densities (same weights as the height) and is carried with it;
-//    the field being linear along the chord, the trapezoid rule on the two
-//    crossings is its exact integral.  This never forms rho_A + grad.(p - A),
-//    which is ill-conditioned in FP32 for sliver tets.  Per-ray and per-pixel
-//    sums stay FP64; tet volumes and vertex densities are the FP64 ones.
-//
-// Per step: one 64 B geometry record + one 16 B record of vertex densities.
+//  * Each face crossing is computed once (as the leave face, height and
+//    density relative to A in FP32, height made absolute in FP64) and
+//    carried as the next tet's entry; errors in a crossing then cancel to
+//    first order because the DTFE field is continuous across the face.  The
+//    field is linear along the chord, so the trapezoid rule on the two
+//    crossings is its exact integral; this never forms
+//    rho_A + grad.(p - A), which is ill-conditioned for sliver tets.
+//  * The ray's first and last crossings (their errors have no neighbour to
+//    cancel against), crossings with ill-conditioned FP32 weights (nearly
+//    vertical faces) and midpoints at the slab boundary are done in FP64.  Per-ray and per-pixel
+//    sums stay FP64; volumes and vertex densities are the FP64 ones.
 #pragma once
 
 #include <algorithm>
@@ -41,59 +44,66 @@ constexpr float kCondMax = 1e-4f;
 
 struct Tet32 {
   int nb0, nb1, nb2, nb3;
-  float ax, ay, az, bx, by, bz, cx, cy, cz, dx, dy, dz;
+  double ax, ay, az;
+  float ebx, eby, ebz, ecx, ecy, ecz, edx, edy, edz;  // B - A, C - A, D - A
+  float ra, rb, rc, rd;                               // vertex densities
 };
 
 SDTFE_HD inline void load_tet32(const Rec32 *__restrict__ rec, int t, Tet32 &r) {
-  const float4 *p = reinterpret_cast<const float4 *>(rec + t);
-  const int4 nb = *reinterpret_cast<const int4 *>(p);
-  const float4 q0 = p[1], q1 = p[2], q2 = p[3];
-  r.nb0 = nb.x; r.nb1 = nb.y; r.nb2 = nb.z; r.nb3 = nb.w;
-  r.ax = q0.x; r.ay = q0.y; r.az = q0.z;
-  r.bx = q0.w; r.by = q1.x; r.bz = q1.y;
-  r.cx = q1.z; r.cy = q1.w; r.cz = q2.x;
-  r.dx = q2.y; r.dy = q2.z; r.dz = q2.w;
+  const uint4 *p = reinterpret_cast<const uint4 *>(rec + t);
+  const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3], q4 = p[4], q5 = p[5];
+  r.nb0 = (int)q0.x; r.nb1 = (int)q0.y; r.nb2 = (int)q0.z; r.nb3 = (int)q0.w;
+  r.ax = __builtin_bit_cast(double, ((uint64_t)q1.y << 32) | q1.x);
+  r.ay = __builtin_bit_cast(double, ((uint64_t)q1.w << 32) | q1.z);
+  r.az = __builtin_bit_cast(double, ((uint64_t)q2.y << 32) | q2.x);
+  r.ebx = __builtin_bit_cast(float, q2.z); r.eby = __builtin_bit_cast(float, q2.w);
+  r.ebz = __builtin_bit_cast(float, q3.x); r.ecx = __builtin_bit_cast(float, q3.y);
+  r.ecy = __builtin_bit_cast(float, q3.z); r.ecz = __builtin_bit_cast(float, q3.w);
+  r.edx = __builtin_bit_cast(float, q4.x); r.edy = __builtin_bit_cast(float, q4.y);
+  r.edz = __builtin_bit_cast(float, q4.z); r.ra = __builtin_bit_cast(float, q4.w);
+  r.rb = __builtin_bit_cast(float, q5.x); r.rc = __builtin_bit_cast(float, q5.y);
+  r.rd = __builtin_bit_cast(float, q5.z);
 }
 
 SDTFE_HD inline int nb32(const Tet32 &r, int f) { return pick4i(f, r.nb0, r.nb1, r.nb2, r.nb3); }
 
-// Per-map constants of the filter (host-computed, see filter_consts()).
+// Per-launch constants of the filter (host-computed, mixed_consts()).
 struct Filter {
-  float e;       // bound on |(X - p) stored - (X - p) exact| per component
-  float k_prod;  // relative rounding of the products and difference
+  float k_prod;  // relative rounding of the two products and their difference
+  float k_off;   // rounding of an offset, relative to the L1 norms it is built from
   float thr;     // (ZERO + FP64 expression error) / lz
 };
 
-// Sign of one side product, and its FP32 value for the barycentric weights.
-// (x1, y1) = X - p, (x2, y2) = Y - p in FP32; l1 = |x|+|y| of each.
+// One side product: FP32 value, its error bound and its sign (0: undecided).
 struct Side {
   float c;
-  float b;  // bound on |c - exact|
+  float b;
   int s;
 };
 
-SDTFE_HD inline Side side32(float x1, float y1, float l1a, float x2, float y2, float l1b,
-                              const Filter &f, bool &need64) {
+// (x1, y1) = X - p, (x2, y2) = Y - p in FP32; l1 = |x| + |y| of each and
+// e1, e2 their rounding bounds (per component).
+SDTFE_HD inline Side side32(float x1, float y1, float l1, float e1, float x2, float y2, float l2,
+                            float e2, const Filter &f) {
   const float p1 = x1 * y2, p2 = y1 * x2;
   const float c = p1 - p2;
-  const float bound = f.k_prod * (fabsf(p1) + fabsf(p2)) + f.e * (l1a + l1b) + f.thr;
+  const float b = f.k_prod * (fabsf(p1) + fabsf(p2)) + e1 * l2 + e2 * l1 + e1 * e2 + f.thr;
   Side s;
   s.c = c;
-  s.b = bound;
-  s.s = c > bound ? 1 : (c < -bound ? -1 : 0);
-  need64 = s.s == 0;
+  s.b = b;
+  s.s = c > b ? 1 : (c < -b ? -1 : 0);
   return s;
 }
 
-// Crossing points of the four faces from FP32 side values: height and
-// density there, both the barycentric mix of the face's three vertices with
-// the weights the FP64 walk uses for the height, written relative to the
-// face's first vertex.  rho holds the vertex densities (A, B, C, D).
-// Straight-line for all faces; callers pick one (see pick4).
 struct Cross {
-  float z, rho;
+  double z;  // absolute height
+  float rho;
 };
 
+// Crossing points of the four faces from FP32 side values: height and
+// density relative to A, each the barycentric mix of the face's three
+// vertices with the weights the FP64 walk uses for the height (face_z);
+// straight-line for all faces, callers pick one (see pick4).
 struct Crosses {
   float z[4], rho[4];
 };
@@ -108,26 +118,33 @@ SDTFE_HD inline void face_point32(float t0, float t1, float t2, float w1n, float
 }
 
 SDTFE_HD inline Crosses face_crosses32(float cCB, float cAC, float cBA, float cDC, float cBD,
-                                       float cDA, const Tet32 &r, const float4 &rho) {
+                                       float cDA, const Tet32 &r) {
   Crosses c;
+  const float za = 0.f, zb = r.ebz, zc = r.ecz, zd = r.edz;
   // BDC: D + w1 (C - D) + w2 (B - D); ACD: C, D, A; ADB: B, A, D; ABC: A, B, C
-  face_point32(-cCB, cDC, -cBD, -cBD, cDC, r.dz, r.cz, r.bz, rho.w, rho.z, rho.y, c.z[0], c.rho[0]);
-  face_point32(cAC, -cDC, -cDA, cAC, -cDC, r.cz, r.dz, r.az, rho.z, rho.w, rho.x, c.z[1], c.rho[1]);
-  face_point32(cDA, cBD, -cBA, cBD, -cBA, r.bz, r.az, r.dz, rho.y, rho.x, rho.w, c.z[2], c.rho[2]);
-  face_point32(cBA, cCB, -cAC, -cAC, cBA, r.az, r.bz, r.cz, rho.x, rho.y, rho.z, c.z[3], c.rho[3]);
+  face_point32(-cCB, cDC, -cBD, -cBD, cDC, zd, zc, zb, r.rd, r.rc, r.rb, c.z[0], c.rho[0]);
+  face_point32(cAC, -cDC, -cDA, cAC, -cDC, zc, zd, za, r.rc, r.rd, r.ra, c.z[1], c.rho[1]);
+  face_point32(cDA, cBD, -cBA, cBD, -cBA, zb, za, zd, r.rb, r.ra, r.rd, c.z[2], c.rho[2]);
+  face_point32(cBA, cCB, -cAC, -cAC, cBA, za, zb, zc, r.ra, r.rb, r.rc, c.z[3], c.rho[3]);
   return c;
 }
 
-SDTFE_HD inline Cross pick_cross(int f, const Crosses &c) {
-  return Cross{pick4f(f, c.z[0], c.z[1], c.z[2], c.z[3]),
+// c ? a : b per lane, without control flow (see pick4).
+SDTFE_HD inline Cross sel_cross(bool c, const Cross &a, const Cross &b) {
+  const int f = c ? 0 : 1;
+  return Cross{pick4(f, a.z, b.z, b.z, b.z), pick4f(f, a.rho, b.rho, b.rho, b.rho)};
+}
+
+SDTFE_HD inline Cross pick_cross(int f, const Crosses &c, double az) {
+  return Cross{az + (double)pick4f(f, c.z[0], c.z[1], c.z[2], c.z[3]),
                pick4f(f, c.rho[0], c.rho[1], c.rho[2], c.rho[3])};
 }
 
-// The same crossing from the FP64 record (side products with the reference
+// The same crossing from the FP64 record: side products with the reference
 // expression, weights and height in FP64; the density mixes the FP32 vertex
-// densities).  Height returned relative to org_z.
+// densities.
 SDTFE_HD inline Cross face_cross64(int f, const Tet64 &q, double lz, double ox, double oy,
-                                   const float4 &rho, double org_z) {
+                                   const Tet32 &r) {
   const double p2 = lz, p3 = -lz * oy, p4 = lz * ox;
   const double uCB = edge_u(p2, p3, p4, q.cx, q.cy, q.bx, q.by);
   const double uAC = edge_u(p2, p3, p4, q.cx, q.cy, q.ax, q.ay);
@@ -135,8 +152,7 @@ SDTFE_HD inline Cross face_cross64(int f, const Tet64 &q, double lz, double ox, 
   const double uDC = edge_u(p2, p3, p4, q.cx, q.cy, q.dx, q.dy);
   const double uBD = edge_u(p2, p3, p4, q.bx, q.by, q.dx, q.dy);
   const double uDA = edge_u(p2, p3, p4, q.dx, q.dy, q.ax, q.ay);
-  double zz[4], rr[4];
-  const double ra = rho.x, rb = rho.y, rc = rho.z, rd = rho.w;
+  const double ra = r.ra, rb = r.rb, rc = r.rc, rd = r.rd;
   const double t[4][5] = {{-uCB, uDC, -uBD, -uBD, uDC},
                           {uAC, -uDC, -uDA, uAC, -uDC},
                           {uDA, uBD, -uBA, uBD, -uBA},
@@ -144,6 +160,7 @@ SDTFE_HD inline Cross face_cross64(int f, const Tet64 &q, double lz, double ox, 
   const double zv[4][3] = {{q.dz, q.cz, q.bz}, {q.cz, q.dz, q.az}, {q.bz, q.az, q.dz},
                            {q.az, q.bz, q.cz}};
   const double rv[4][3] = {{rd, rc, rb}, {rc, rd, ra}, {rb, ra, rd}, {ra, rb, rc}};
+  double zz[4], rr[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const double inv = 1.0 / (t[k][0] + t[k][1] + t[k][2]);
@@ -151,10 +168,7 @@ SDTFE_HD inline Cross face_cross64(int f, const Tet64 &q, double lz, double ox, 
     zz[k] = zv[k][0] + w1 * (zv[k][1] - zv[k][0]) + w2 * (zv[k][2] - zv[k][0]);
     rr[k] = rv[k][0] + w1 * (rv[k][1] - rv[k][0]) + w2 * (rv[k][2] - rv[k][0]);
   }
-  Cross c;
-  c.z = (float)(pick4(f, zz[0], zz[1], zz[2], zz[3]) - org_z);
-  c.rho = (float)pick4(f, rr[0], rr[1], rr[2], rr[3]);
-  return c;
+  return Cross{pick4(f, zz[0], zz[1], zz[2], zz[3]), (float)pick4(f, rr[0], rr[1], rr[2], rr[3])};
 }
 
 // Relative uncertainty of face f's FP32 barycentric weights.
@@ -169,7 +183,7 @@ SDTFE_HD inline float face_cond(int f, const Side &CB, const Side &AC, const Sid
 
 // Face choice from the six side signs (src/dtfe.c:64-213, as plucker64).
 SDTFE_HD inline void choose_faces(int sCB, int sAC, int sBA, int sDC, int sBD, int sDA,
-                                    int &enter, int &leave, bool &ok, bool &special) {
+                                  int &enter, int &leave, bool &ok, bool &special) {
   const int fa[4] = {-sCB, sAC, sDA, sBA};
   const int fb[4] = {sDC, -sDC, sBD, sCB};
   const int fc[4] = {-sBD, -sDA, -sBA, -sAC};
@@ -192,20 +206,19 @@ SDTFE_HD inline void choose_faces(int sCB, int sAC, int sBA, int sDC, int sBD, i
 }
 
 // One sample-ray.  (x, y) FP64 sample position; the walk keeps the FP64
-// position (changed only by perturbations) and its FP32 offset from the
-// origin.
-SDTFE_HD inline double walk_mixed(const Rec32 *__restrict__ rec32, const Field32 *__restrict__ fld,
-                             const Rec64 *__restrict__ rec64, int entry, double x, double y,
-                             const MapGeom &g, const Filter &flt, float zlo, float zhi,
-                             float zband, uint64_t key, int &crossed, LaneCounters &cnt) {
+// position, which only a perturbation changes.
+SDTFE_HD inline double walk_mixed(const Rec32 *__restrict__ rec32,
+                                  const Rec64 *__restrict__ rec64, int entry, double x,
+                                  double y, const MapGeom &g, const Filter &flt, uint64_t key,
+                                  int &crossed, LaneCounters &cnt) {
   crossed = 0;
   if (entry < 0) return 0.0;
   const double step = (g.dxy / 2.0) / kPerturbMax;
+  const double z_lo = g.off_z, z_hi = g.off_z + g.lz;
   double ox = x, oy = y, acc = 0.0;
-  float rx = (float)(x - g.org_x), ry = (float)(y - g.org_y);
   uint32_t draw = 2;
   int cur = entry, prev = -1, f_in = -1, f_out = 0, nxt = entry, steps = 0;
-  Cross carry{0.f, 0.f};
+  Cross carry{0.0, 0.f};
   bool have_carry = false;
   Tet32 r;
   for (;;) {
@@ -227,40 +240,48 @@ SDTFE_HD inline double walk_mixed(const Rec32 *__restrict__ rec32, const Field32
     prev = cur;
     cur = nxt;
     load_tet32(rec32, cur, r);
-    const float4 rho4 = *reinterpret_cast<const float4 *>(fld + cur);
 
-    // vertex offsets from the ray, FP32
-    const float ax = r.ax - rx, ay = r.ay - ry, bx = r.bx - rx, by = r.by - ry;
-    const float cx = r.cx - rx, cy = r.cy - ry, dx = r.dx - rx, dy = r.dy - ry;
+    // offsets of the vertices from the ray, FP32, relative to A
+    const float ax = (float)(r.ax - ox), ay = (float)(r.ay - oy);
+    const float bx = ax + r.ebx, by = ay + r.eby, cx = ax + r.ecx, cy = ay + r.ecy;
+    const float dx = ax + r.edx, dy = ay + r.edy;
     const float la = fabsf(ax) + fabsf(ay), lb = fabsf(bx) + fabsf(by);
     const float lc = fabsf(cx) + fabsf(cy), ld = fabsf(dx) + fabsf(dy);
-    bool n0, n1, n2, n3, n4, n5;
-    Side CB = side32(cx, cy, lc, bx, by, lb, flt, n0);
-    Side AC = side32(cx, cy, lc, ax, ay, la, flt, n1);
-    Side BA = side32(bx, by, lb, ax, ay, la, flt, n2);
-    Side DC = side32(cx, cy, lc, dx, dy, ld, flt, n3);
-    Side BD = side32(bx, by, lb, dx, dy, ld, flt, n4);
-    Side DA = side32(dx, dy, ld, ax, ay, la, flt, n5);
-    if (n0 | n1 | n2 | n3 | n4 | n5) {
+    // rounding bounds per component: A - p once, the others also carry the
+    // stored edge and the sum
+    const float ea = flt.k_off * la;
+    const float eb = flt.k_off * (la + fabsf(r.ebx) + fabsf(r.eby) + lb);
+    const float ec = flt.k_off * (la + fabsf(r.ecx) + fabsf(r.ecy) + lc);
+    const float ed = flt.k_off * (la + fabsf(r.edx) + fabsf(r.edy) + ld);
+    Side CB = side32(cx, cy, lc, ec, bx, by, lb, eb, flt);
+    Side AC = side32(cx, cy, lc, ec, ax, ay, la, ea, flt);
+    Side BA = side32(bx, by, lb, eb, ax, ay, la, ea, flt);
+    Side DC = side32(cx, cy, lc, ec, dx, dy, ld, ed, flt);
+    Side BD = side32(bx, by, lb, eb, dx, dy, ld, ed, flt);
+    Side DA = side32(dx, dy, ld, ed, ax, ay, la, ea, flt);
+    const bool undecided =
+        (CB.s == 0) | (AC.s == 0) | (BA.s == 0) | (DC.s == 0) | (BD.s == 0) | (DA.s == 0);
+    if (undecided) {
       // undecided sides: the reference expression in FP64
       Tet64 q;
       load_tet(rec64, cur, q);
       const double p2 = g.lz, p3 = -g.lz * oy, p4 = g.lz * ox;
       const float il = (float)(1.0 / g.lz);
       unsigned redo = 0;
-#define SDTFE_REDO(S, N, XX, XY, YX, YY)                                 \
-  if (N) {                                                               \
+#define SDTFE_REDO(S, XX, XY, YX, YY)                                    \
+  if (S.s == 0) {                                                        \
     const double u = edge_u(p2, p3, p4, q.XX, q.XY, q.YX, q.YY);         \
     S.s = sign_zero(u);                                                  \
     S.c = (float)u * il;                                                 \
+    S.b = 0.f;                                                           \
     ++redo;                                                              \
   }
-      SDTFE_REDO(CB, n0, cx, cy, bx, by)
-      SDTFE_REDO(AC, n1, cx, cy, ax, ay)
-      SDTFE_REDO(BA, n2, bx, by, ax, ay)
-      SDTFE_REDO(DC, n3, cx, cy, dx, dy)
-      SDTFE_REDO(BD, n4, bx, by, dx, dy)
-      SDTFE_REDO(DA, n5, dx, dy, ax, ay)
+      SDTFE_REDO(CB, cx, cy, bx, by)
+      SDTFE_REDO(AC, cx, cy, ax, ay)
+      SDTFE_REDO(BA, bx, by, ax, ay)
+      SDTFE_REDO(DC, cx, cy, dx, dy)
+      SDTFE_REDO(BD, bx, by, dx, dy)
+      SDTFE_REDO(DA, dx, dy, ax, ay)
 #undef SDTFE_REDO
       cnt.fp64_redo += redo;
     }
@@ -281,8 +302,6 @@ SDTFE_HD inline double walk_mixed(const Rec32 *__restrict__ rec32, const Field32
         crossed = 0;
         return 0.0;
       }
-      rx = (float)(ox - g.org_x);
-      ry = (float)(oy - g.org_y);
       f_in = h.enter;
       f_out = h.leave;
       ++crossed;
@@ -293,67 +312,66 @@ SDTFE_HD inline double walk_mixed(const Rec32 *__restrict__ rec32, const Field32
     f_in = enter;
     f_out = leave;
     ++crossed;
-    // A crossing is taken in FP64 when its error would not cancel against a
-    // neighbour's (the ray's first and last crossing) or when the face's FP32
-    // weights are ill-conditioned (nearly vertical faces).
+    const Crosses cr = face_crosses32(CB.c, AC.c, BA.c, DC.c, BD.c, DA.c, r);
+    const Cross fresh = pick_cross(enter, cr, r.az);
+    Cross in = sel_cross(have_carry, carry, fresh);
+    Cross out = pick_cross(leave, cr, r.az);
+    // FP64 for the crossings whose error no neighbour cancels (where the ray
+    // enters and leaves the hull) and for ill-conditioned weights
     const bool in64 = !have_carry;
-    const bool out64 = nb32(r, leave) == -1 ||
-                       face_cond(leave, CB, AC, BA, DC, BD, DA) > kCondMax;
-    const Crosses cr = face_crosses32(CB.c, AC.c, BA.c, DC.c, BD.c, DA.c, r, rho4);
-    Cross in = carry, out = pick_cross(leave, cr);
+    const bool out64 =
+        nb32(r, leave) == -1 || face_cond(leave, CB, AC, BA, DC, BD, DA) > kCondMax;
     if (in64 | out64) {
       Tet64 q;
       load_tet(rec64, cur, q);
-      if (in64) in = face_cross64(enter, q, g.lz, ox, oy, rho4, g.org_z);
-      if (out64) out = face_cross64(leave, q, g.lz, ox, oy, rho4, g.org_z);
+      const Cross in_64 = face_cross64(enter, q, g.lz, ox, oy, r);
+      const Cross out_64 = face_cross64(leave, q, g.lz, ox, oy, r);
+      in = sel_cross(in64, in_64, in);
+      out = sel_cross(out64, out_64, out);
       cnt.fp64_redo++;
     }
     carry = out;
     have_carry = true;
-    const float dz = out.z - in.z;
-    const float zm = fmaf(dz, 0.5f, in.z);
-    bool inside = zm > zlo && zm < zhi;
-    if (fabsf(zm - zlo) < zband || fabsf(zm - zhi) < zband) {
-      // near the slab boundary: the FP64 walk's own midpoint decides
+    const double dz = out.z - in.z;
+    const double zm = in.z + 0.5 * dz;
+    bool inside = zm > z_lo && zm < z_hi;
+    const double band = 1e-4 * fabs(dz) + 1e-12 * g.lz;
+    if (fabs(zm - z_lo) < band || fabs(zm - z_hi) < band) {
+      // at the slab boundary: the FP64 walk's own midpoint decides
       Tet64 q;
       load_tet(rec64, cur, q);
       const Hit h = plucker64(q, g.lz, ox, oy);
       const double dz64 = h.z_out - h.z_in;
       const double zm64 = dz64 / 2 + h.z_in;
-      inside = zm64 > g.off_z && zm64 < g.off_z + g.lz;
+      inside = zm64 > z_lo && zm64 < z_hi;
       cnt.fp64_redo++;
     }
     // the field is linear along the chord: trapezoid = exact integral
-    if (inside) acc += (double)(0.5f * (in.rho + out.rho) * dz);
+    if (inside) acc += (double)(0.5f * (in.rho + out.rho)) * dz;
   }
   return acc;
 }
 
 struct MixedConsts {
   Filter flt;
-  float zlo, zhi, zband;
 };
 
-// Filter constants for a launch: stored vertices and rays lie within `span`
-// of the origin (per coordinate).
+// Filter constants for a launch.
 inline MixedConsts mixed_consts(const sdtfe_tess *t, const MapGeom &g) {
   const double u32 = 1.0 / (1 << 24);
-  const double span = std::max(t->span, g.ray_span);
   // largest absolute coordinate the FP64 expression sees
-  const double amax = span + std::max(std::fabs(g.org_x), std::fabs(g.org_y));
+  const double amax = std::max(t->span + std::max(std::fabs(t->org[0]), std::fabs(t->org[1])),
+                               g.ray_span + std::max(std::fabs(g.org_x), std::fabs(g.org_y)));
   MixedConsts m;
-  // stored vertex and ray each carry u32*span; their FP32 difference adds
-  // u32 * 2 span: E = 4 u32 span, padded for the rounding of the bound itself
-  const double e = 4.25 * u32 * span;
-  m.flt.e = (float)e;
   m.flt.k_prod = (float)(3.0 * u32);
-  // reference zero band, the FP64 expression's own rounding (three products
-  // and two sums of terms below 2 lz amax^2) and the 2 E^2 cross term
+  // each offset is a rounded FP64 difference, a rounded stored edge and a
+  // rounded sum: within u32 of the L1 norms involved, padded for the
+  // rounding of the bound itself
+  m.flt.k_off = (float)(1.01 * u32);
+  // reference zero band and the FP64 expression's own rounding (three
+  // products and two sums of terms below 2 lz amax^2), over lz
   const double err64 = 16.0 * std::ldexp(1.0, -53) * 2.0 * g.lz * amax * amax;
-  m.flt.thr = (float)(1.01 * ((1e4 * 2.220446049250313080847e-16 + err64) / g.lz + 2 * e * e));
-  m.zlo = (float)(g.off_z - g.org_z);
-  m.zhi = (float)(g.off_z + g.lz - g.org_z);
-  m.zband = (float)(1e-5 * std::max(g.lz, span));
+  m.flt.thr = (float)(1.01 * (1e4 * 2.220446049250313080847e-16 + err64) / g.lz);
   return m;
 }
 

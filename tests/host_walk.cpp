@@ -30,13 +30,10 @@ int main(int argc, char **argv) {
       fread(ent.data(), 8, (size_t)n, f) != (size_t)n)
     return 6;
   std::vector<sdtfe::Rec32> rec32;
-  std::vector<sdtfe::Field32> fld;
   double org_span[4] = {0, 0, 0, 0};
   if (mixed) {
     rec32.resize((size_t)nt);
-    fld.resize((size_t)nt);
     if (fread(rec32.data(), sizeof(sdtfe::Rec32), (size_t)nt, f) != (size_t)nt ||
-        fread(fld.data(), sizeof(sdtfe::Field32), (size_t)nt, f) != (size_t)nt ||
         fread(org_span, 8, 4, f) != 4)
       return 8;
   }
@@ -52,6 +49,9 @@ int main(int argc, char **argv) {
   g.org_z = org_span[2];
   sdtfe_tess tt;
   tt.span = org_span[3];
+  tt.org[0] = org_span[0];
+  tt.org[1] = org_span[1];
+  tt.org[2] = org_span[2];
   for (long long i = 0; i < n; ++i)
     g.ray_span = std::max(g.ray_span, std::max(std::fabs(x[i] - g.org_x), std::fabs(y[i] - g.org_y)));
   const sdtfe::MixedConsts mc = sdtfe::mixed_consts(&tt, g);
@@ -63,8 +63,8 @@ int main(int argc, char **argv) {
     int crossed = 0;
     const uint64_t key = sdtfe::sample_key(seed, pix[i], (uint32_t)smp[i]);
     const double v =
-        mixed ? sdtfe::walk_mixed(rec32.data(), fld.data(), rec.data(), (int)ent[i], x[i], y[i],
-                                  g, mc.flt, mc.zlo, mc.zhi, mc.zband, key, crossed, c)
+        mixed ? sdtfe::walk_mixed(rec32.data(), rec.data(), (int)ent[i], x[i], y[i], g, mc.flt,
+                                  key, crossed, c)
               : sdtfe::walk64(rec.data(), (int)ent[i], x[i], y[i], g, key, crossed, c);
     redo += c.fp64_redo;
     const double cc = crossed;

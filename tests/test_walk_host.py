@@ -36,13 +36,19 @@ def records(pos, tetra, rho, nabla):
     return rec
 
 
-def records32(pos, tetra, rho, nabla, org):
+def records32(pos, tetra, rho):
+    """The mixed walk's 96-byte records: neighbours, A (FP64), B-A, C-A, D-A
+    and the vertex densities (FP32)."""
     t = tetra.shape[0]
-    rec = np.zeros(t, dtype=np.dtype([("nb", "<i4", 4), ("v", "<f4", 12)]))
+    rec = np.zeros(t, dtype=np.dtype([("nb", "<i4", 4), ("a", "<f8", 3), ("e", "<f4", 9),
+                                      ("rho", "<f4", 4), ("pad", "<f4")]))
+    assert rec.dtype.itemsize == 96
+    p = pos[tetra[:, :4]]
     rec["nb"] = tetra[:, 4:]
-    rec["v"] = (pos[tetra[:, :4]] - org).astype(np.float32).reshape(t, 12)
-    fld = rho[tetra[:, :4]].astype(np.float32)
-    return rec, fld
+    rec["a"] = p[:, 0]
+    rec["e"] = (p[:, 1:] - p[:, :1]).astype(np.float32).reshape(t, 9)
+    rec["rho"] = rho[tetra[:, :4]].astype(np.float32)
+    return rec
 
 
 def run_walk(host_walk, tmp_path, g, mixed=False):
@@ -61,9 +67,7 @@ def run_walk(host_walk, tmp_path, g, mixed=False):
         if mixed:
             lo, hi = g["pos"].min(0), g["pos"].max(0)
             org = 0.5 * (lo + hi)
-            r32, f32 = records32(g["pos"], g["tetra"], g["rho"], g["nabla"], org)
-            r32.tofile(f)
-            f32.tofile(f)
+            records32(g["pos"], g["tetra"], g["rho"]).tofile(f)
             np.array([*org, float((0.5 * (hi - lo)).max())]).tofile(f)
     r = subprocess.run([host_walk, str(inp), str(out)] + (["mixed"] if mixed else []),
                        check=True, capture_output=True, text=True)
