@@ -105,13 +105,22 @@ struct Cross {
 // vertices with the weights the FP64 walk uses for the height (face_z);
 // straight-line for all faces, callers pick one (see pick4).
 struct Crosses {
-  float z[4], rho[4];
+  float z[4], rho[4], inv[4];  // inv: 1 / (sum of the face's signed sides)
 };
+
+// Reciprocal to within 1 ulp (v_rcp_f32) on the device.
+SDTFE_HD inline float rcp32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
 
 SDTFE_HD inline void face_point32(float t0, float t1, float t2, float w1n, float w2n, float z0,
                                   float z1, float z2, float q0, float q1, float q2, float &z,
-                                  float &q) {
-  const float inv = 1.0f / (t0 + t1 + t2);
+                                  float &q, float &inv) {
+  inv = rcp32(t0 + t1 + t2);
   const float w1 = w1n * inv, w2 = w2n * inv;
   z = fmaf(w1, z1 - z0, fmaf(w2, z2 - z0, z0));
   q = fmaf(w1, q1 - q0, fmaf(w2, q2 - q0, q0));
@@ -122,10 +131,14 @@ SDTFE_HD inline Crosses face_crosses32(float cCB, float cAC, float cBA, float cD
   Crosses c;
   const float za = 0.f, zb = r.ebz, zc = r.ecz, zd = r.edz;
   // BDC: D + w1 (C - D) + w2 (B - D); ACD: C, D, A; ADB: B, A, D; ABC: A, B, C
-  face_point32(-cCB, cDC, -cBD, -cBD, cDC, zd, zc, zb, r.rd, r.rc, r.rb, c.z[0], c.rho[0]);
-  face_point32(cAC, -cDC, -cDA, cAC, -cDC, zc, zd, za, r.rc, r.rd, r.ra, c.z[1], c.rho[1]);
-  face_point32(cDA, cBD, -cBA, cBD, -cBA, zb, za, zd, r.rb, r.ra, r.rd, c.z[2], c.rho[2]);
-  face_point32(cBA, cCB, -cAC, -cAC, cBA, za, zb, zc, r.ra, r.rb, r.rc, c.z[3], c.rho[3]);
+  face_point32(-cCB, cDC, -cBD, -cBD, cDC, zd, zc, zb, r.rd, r.rc, r.rb, c.z[0], c.rho[0],
+               c.inv[0]);
+  face_point32(cAC, -cDC, -cDA, cAC, -cDC, zc, zd, za, r.rc, r.rd, r.ra, c.z[1], c.rho[1],
+               c.inv[1]);
+  face_point32(cDA, cBD, -cBA, cBD, -cBA, zb, za, zd, r.rb, r.ra, r.rd, c.z[2], c.rho[2],
+               c.inv[2]);
+  face_point32(cBA, cCB, -cAC, -cAC, cBA, za, zb, zc, r.ra, r.rb, r.rc, c.z[3], c.rho[3],
+               c.inv[3]);
   return c;
 }
 
@@ -171,14 +184,15 @@ SDTFE_HD inline Cross face_cross64(int f, const Tet64 &q, double lz, double ox, 
   return Cross{pick4(f, zz[0], zz[1], zz[2], zz[3]), (float)pick4(f, rr[0], rr[1], rr[2], rr[3])};
 }
 
-// Relative uncertainty of face f's FP32 barycentric weights.
+// Relative uncertainty of face f's FP32 barycentric weights: the summed
+// bounds of its sides over the magnitude of their signed sum.
 SDTFE_HD inline float face_cond(int f, const Side &CB, const Side &AC, const Side &BA,
-                                const Side &DC, const Side &BD, const Side &DA) {
-  const float c0 = (CB.b + DC.b + BD.b) / fabsf(-CB.c + DC.c - BD.c);
-  const float c1 = (AC.b + DC.b + DA.b) / fabsf(AC.c - DC.c - DA.c);
-  const float c2 = (DA.b + BD.b + BA.b) / fabsf(DA.c + BD.c - BA.c);
-  const float c3 = (BA.b + CB.b + AC.b) / fabsf(BA.c + CB.c - AC.c);
-  return pick4f(f, c0, c1, c2, c3);
+                                const Side &DC, const Side &BD, const Side &DA,
+                                const Crosses &cr) {
+  const float b0 = CB.b + DC.b + BD.b, b1 = AC.b + DC.b + DA.b;
+  const float b2 = DA.b + BD.b + BA.b, b3 = BA.b + CB.b + AC.b;
+  return pick4f(f, b0 * fabsf(cr.inv[0]), b1 * fabsf(cr.inv[1]), b2 * fabsf(cr.inv[2]),
+                b3 * fabsf(cr.inv[3]));
 }
 
 // Face choice from the six side signs (src/dtfe.c:64-213, as plucker64).
@@ -320,7 +334,7 @@ SDTFE_HD inline double walk_mixed(const Rec32 *__restrict__ rec32,
     // enters and leaves the hull) and for ill-conditioned weights
     const bool in64 = !have_carry;
     const bool out64 =
-        nb32(r, leave) == -1 || face_cond(leave, CB, AC, BA, DC, BD, DA) > kCondMax;
+        nb32(r, leave) == -1 || face_cond(leave, CB, AC, BA, DC, BD, DA, cr) > kCondMax;
     if (in64 | out64) {
       Tet64 q;
       load_tet(rec64, cur, q);
