@@ -73,11 +73,12 @@ def time_form(tess, cfg, precision, steps, warmup, seed):
         tess.map_device(p, buf, stats=True)
     buf.zero()
     tess.sync()
-    walk_ms, steps_per_map, st_last = [], [], None
+    walk_ms, kernel_ms, steps_per_map, st_last = [], [], [], None
     t0 = time.perf_counter()
     for _ in range(steps):
         st = tess.map_device(p, buf, stats=True)
         walk_ms.append(st["t_walk_ms"])
+        kernel_ms.append(st["t_kernel_ms"])
         steps_per_map.append(st["tet_steps"])
         st_last = st
     tess.sync()
@@ -89,7 +90,20 @@ def time_form(tess, cfg, precision, steps, warmup, seed):
     acc = buf.to_host()
     buf.close()
     return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, walk_ms=float(np.mean(walk_ms)),
-                tet_steps=int(np.mean(steps_per_map)), stats=st_last, map=acc)
+                kernel_ms=float(np.mean(kernel_ms)), tet_steps=int(np.mean(steps_per_map)),
+                stats=st_last, map=acc)
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), each
+    scaled by the same counter's reading on a copy of known size; None when
+    that workload was not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(kernel, {}).get(workload)
 
 
 def cpu_baseline(pos, tetra, cfg, seed, target_s, gpu_map):
@@ -224,11 +238,24 @@ def main():
     f = forms[head]
     rays = G * G * S
     value = rays / (f["ms_per_step"] / 1e3)
-    achieved = f["tet_steps"] * BYTES_PER_STEP / (f["walk_ms"] / 1e3) / 1e9
+    kernel = "map_mixed_kernel" if head == "mixed" else "map_fp64_kernel"
+    workload = f"{a.config}-{pos.shape[0]} {G}x{G} map, {S} samples/pixel"
+    achieved = f["tet_steps"] * BYTES_PER_STEP / (f["kernel_ms"] / 1e3) / 1e9
+    tr = measured_traffic(kernel, workload)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel=f"map_{head}_kernel", bytes_per_step=BYTES_PER_STEP,
-                    tet_steps_per_launch=f["tet_steps"], walk_ms=round(f["walk_ms"], 3))
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=None if tr is None else tr["bytes_per_launch"],
+                    kernel=kernel, bytes_per_step=BYTES_PER_STEP,
+                    tet_steps_per_launch=f["tet_steps"], kernel_ms=round(f["kernel_ms"], 3),
+                    walk_ms=round(f["walk_ms"], 3))
+    if tr is not None:
+        phys = tr["bytes_per_launch"] / (f["kernel_ms"] / 1e3) / 1e9
+        roofline["physical_gbs"] = round(phys, 1)
+        roofline["reuse"] = round(achieved / phys, 2)
+    if achieved > HBM_PEAK_GBS:
+        # cache reuse between neighbouring rays lifts the algorithmic rate
+        # past the HBM peak: the kernel is not HBM-bound
+        roofline["hbm_bound"] = False
 
     cpu = None
     if not a.no_cpu:
@@ -245,7 +272,7 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{a.config}-{pos.shape[0]} {G}x{G} map, {S} samples/pixel",
+        "config": {"workload": workload,
                    "particles": int(pos.shape[0]), "tets": int(tetra.shape[0]),
                    "grid": G, "samples_per_pixel": S, "sample_rays": rays,
                    "precision": head},
@@ -255,6 +282,7 @@ def main():
         "forms": {k: {"sample_rays_per_s": round(rays / (v["ms_per_step"] / 1e3), 1),
                       "ms_per_step": round(v["ms_per_step"], 3),
                       "walk_ms": round(v["walk_ms"], 3),
+                      "kernel_ms": round(v["kernel_ms"], 3),
                       "tet_steps": v["tet_steps"],
                       "missed": v["stats"]["missed"], "failed": v["stats"]["failed"],
                       "perturb_steps": v["stats"]["perturb_steps"],

@@ -97,9 +97,10 @@ typedef struct {
   int64_t perturb_steps;     /* perturbations taken */
   int64_t tet_steps;         /* tets crossed by all rays */
   int64_t step_cap;          /* rays cut by max_steps */
-  int64_t fp64_redo;         /* mixed walk: edge products redone in FP64 */
-  double t_walk_ms;          /* walk kernel time (HIP events) */
+  int64_t fp64_redo;         /* mixed walk: sample-rays handed to the FP64 walk */
+  double t_walk_ms;          /* walk time (HIP events; both passes of a mixed map) */
   double t_total_ms;         /* whole call, host wall clock */
+  double t_kernel_ms;        /* the walk kernel alone (a mixed map's first pass) */
 } sdtfe_stats;
 
 typedef struct {

@@ -34,7 +34,7 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "samples", "missed", "failed", "no_intersection", "perturb_exhausted",
         "no_isect_perturb", "perturb_steps", "tet_steps", "step_cap", "fp64_redo")] + [
-        ("t_walk_ms", C.c_double), ("t_total_ms", C.c_double)]
+        ("t_walk_ms", C.c_double), ("t_total_ms", C.c_double), ("t_kernel_ms", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

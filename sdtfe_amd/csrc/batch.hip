@@ -23,7 +23,7 @@
 #include <vector>
 
 #include "sdtfe_host.h"
-#include "walk_mixed_core.h"
+#include "walk_defer.h"
 
 namespace sdtfe {
 
@@ -50,6 +50,7 @@ struct BatchHdr {
   int n_items;
   int total_tiles;
   int share[kXcd + 1];  // tile-space cut points
+  DeferQueue dq;        // mixed walk: pixels for the FP64 pass (count: counters[kXcd])
 };
 
 // Wave sub-tile shape for L lanes per pixel: 64/L pixels as bw x bh.
@@ -67,15 +68,46 @@ __global__ void batch_reset_kernel(const BatchHdr *hdr, int *counters,
                                    unsigned long long *item_stats, int n_stats) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kXcd) counters[i] = hdr->share[i];
+  if (i == kXcd) counters[kXcd] = 0;
   for (int k = i; k < n_stats; k += gridDim.x * blockDim.x) item_stats[k] = 0ull;
 }
 
+// Per-sample slots in LDS (lanes per pixel > 1): the value, or one of two
+// NaN codes for a sample that adds nothing (a miss or a failed walk) and one
+// handed to the FP64 walk.
+constexpr uint64_t kSlotNone = 0x7ff8dead00000001ull;
+constexpr uint64_t kSlotHanded = 0x7ff8dead00000002ull;
+
+// One sample of a pixel: value and tet count, false if handed over.
 template <bool kMixed>
-__device__ inline double walk_one(const BatchItem &it, int entry, double x, double y,
-                                  uint64_t key, int &crossed, LaneCounters &cnt) {
-  if (kMixed)
-    return walk_mixed(it.rec32, it.rec64, entry, x, y, it.geom, it.mc.flt, key, crossed, cnt);
-  return walk64(it.rec64, entry, x, y, it.geom, key, crossed, cnt);
+__device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int px, int py,
+                                  uint64_t pix, int s, double &v, int &crossed,
+                                  LaneCounters &cnt) {
+  const MapGeom &g = it.geom;
+  const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
+  double xt, yt;
+  sample_point(g, px, py, key, xt, yt);
+  cnt.samples++;
+  v = 0.0;
+  crossed = 0;
+  const int entry = locate_entry(it.grid, xt, yt);
+  if (entry < 0) {
+    cnt.missed++;
+    return true;
+  }
+  if (kMixed) {
+    const bool done = it.mc.flt.full_slab
+                          ? walk_mixed<true>(it.rec32, it.rec64, faces, entry, xt, yt, g,
+                                             it.mc.flt, v, crossed, cnt)
+                          : walk_mixed<false>(it.rec32, it.rec64, faces, entry, xt, yt, g,
+                                              it.mc.flt, v, crossed, cnt);
+    if (!done) return false;
+  } else {
+    v = walk64(it.rec64, entry, xt, yt, g, key, crossed, cnt);
+  }
+  cnt.tet_steps += (unsigned)crossed;
+  if (crossed == 0) cnt.failed++;
+  return true;
 }
 
 template <bool kMixed>
@@ -85,6 +117,8 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchItem *__restrict_
                                                     unsigned long long *item_stats) {
   extern __shared__ double lds[];  // per-sample values for lanes-per-pixel > 1
   __shared__ int s_tile;
+  __shared__ uint8_t faces[kFaceCodes];
+  if (kMixed && threadIdx.x < kFaceCodes) faces[threadIdx.x] = (uint8_t)face_code((int)threadIdx.x);
   const int home = blockIdx.x % kXcd;
   const int n_items = hdr->n_items;
   int xcd = home, tried = 0;
@@ -125,43 +159,83 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchItem *__restrict_
     const bool active = px < g.grid && py < g.grid;
     const uint64_t pix = (uint64_t)py * (uint64_t)g.grid + (uint64_t)px;
     LaneCounters cnt;
-    double sum = 0.0;
-    int n_ok = 0;
+    PixelSamples<1> ps(g.samples);
     const int slot = threadIdx.x / L;  // pixel slot in the block
     if (active) {
       for (int s = j; s < g.samples; s += L) {
-        const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
-        double xt, yt;
-        sample_point(g, px, py, key, xt, yt);
-        cnt.samples++;
-        const int entry = locate_entry(it.grid, xt, yt);
-        double v = 0.0;
-        int crossed = 0;
-        if (entry < 0) {
-          cnt.missed++;
-        } else {
-          v = walk_one<kMixed>(it, entry, xt, yt, key, crossed, cnt);
-          cnt.tet_steps += (unsigned)crossed;
-          if (crossed == 0) cnt.failed++;
-        }
+        double v;
+        int crossed;
+        const bool done = sample_one<kMixed>(it, faces, px, py, pix, s, v, crossed, cnt);
         if (L == 1) {
-          sum += v;  // a miss or a failed walk adds 0 (src/dtfe.c:626-636)
+          if (done)
+            ps.add(v, crossed);  // a miss or a failed walk adds 0 (src/dtfe.c:626-636)
+          else if (!ps.hand_over(s))
+            break;
         } else {
-          lds[slot * g.samples + s] = v;
+          lds[slot * g.samples + s] =
+              done ? (crossed > 0 ? v : __builtin_bit_cast(double, kSlotNone))
+                   : __builtin_bit_cast(double, kSlotHanded);
         }
-        n_ok += crossed > 0;
       }
     }
     if (L > 1) {
-      // n_avg: lanes of one pixel are adjacent within the wave
-      for (int off = 1; off < L; off <<= 1) n_ok += __shfl_xor(n_ok, off, 64);
+      // lanes of one pixel are adjacent within the wave; the first sums in
+      // sample order (a window of kDeferWindow >= 16 L samples fits)
       __syncthreads();
-      if (active && j == 0)
-        for (int s = 0; s < g.samples; ++s) sum += lds[slot * g.samples + s];
+      if (active && j == 0) {
+        PixelSamples<4> pw(g.samples);
+        for (int s = 0; s < g.samples; ++s) {
+          const double v = lds[slot * g.samples + s];
+          const uint64_t u = __builtin_bit_cast(uint64_t, v);
+          if (u == kSlotHanded)
+            pw.hand_over(s);
+          else if (u != kSlotNone)
+            pw.add(v, 1);
+        }
+        if (pw.clean(g.samples))
+          it.out[pix] += pw.n_ok > 0 ? pw.sum / pw.n_ok : 0.0;
+        else
+          push_deferred(hdr->dq, pw, pix, lo);
+      }
+    } else if (active) {
+      if (ps.clean(g.samples))
+        it.out[pix] += ps.n_ok > 0 ? ps.sum / ps.n_ok : 0.0;
+      else
+        push_deferred(hdr->dq, ps, pix, lo);
     }
-    if (active && j == 0) it.out[pix] += n_ok > 0 ? sum / n_ok : 0.0;
     cnt.flush(item_stats + (size_t)lo * ST_COUNT);
     __syncthreads();
+  }
+}
+
+// Second pass of a mixed batch: grid-stride over the queued pixels of every
+// item; counters go straight to the item's stats (lanes of a wave serve
+// different items).
+__global__ __launch_bounds__(256) void batch_deferred_kernel(const BatchItem *__restrict__ items,
+                                                            const BatchHdr *__restrict__ hdr,
+                                                            const int *counters,
+                                                            unsigned long long *item_stats) {
+  const DeferQueue dq = hdr->dq;
+  const int n = min(counters[kXcd], dq.cap);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const Deferred *d = dq.q + i;
+    const BatchItem &it = items[d->item];
+    LaneCounters cnt;
+    int n_ok;
+    const double sum = finish_deferred(d, it.rec64, it.grid, it.geom, n_ok, cnt);
+    it.out[d->pix] += n_ok > 0 ? sum / n_ok : 0.0;
+    unsigned long long *st = item_stats + (size_t)d->item * ST_COUNT;
+    if (cnt.samples) atomicAdd(st + ST_SAMPLES, cnt.samples);
+    if (cnt.missed) atomicAdd(st + ST_MISSED, (unsigned long long)cnt.missed);
+    if (cnt.failed) atomicAdd(st + ST_FAILED, (unsigned long long)cnt.failed);
+    if (cnt.no_isect) atomicAdd(st + ST_NO_ISECT, (unsigned long long)cnt.no_isect);
+    if (cnt.pert_exh) atomicAdd(st + ST_PERT_EXHAUSTED, (unsigned long long)cnt.pert_exh);
+    if (cnt.no_isect_pert)
+      atomicAdd(st + ST_NO_ISECT_PERT, (unsigned long long)cnt.no_isect_pert);
+    if (cnt.pert_steps) atomicAdd(st + ST_PERT_STEPS, (unsigned long long)cnt.pert_steps);
+    if (cnt.tet_steps) atomicAdd(st + ST_TET_STEPS, cnt.tet_steps);
+    if (cnt.step_cap) atomicAdd(st + ST_STEP_CAP, (unsigned long long)cnt.step_cap);
+    if (cnt.fp64_redo) atomicAdd(st + ST_FP64_REDO, (unsigned long long)cnt.fp64_redo);
   }
 }
 
@@ -186,6 +260,8 @@ struct sdtfe_batch {
   int graph_blocks = 0;
   size_t graph_lds = 0;
   int blocks_fp64 = 0, blocks_mixed = 0;
+  sdtfe::Deferred *d_defer = nullptr;  // mixed walk: pixels for the FP64 pass
+  size_t defer_cap = 0;
 };
 
 namespace sdtfe {
@@ -240,6 +316,9 @@ int capture(sdtfe_batch *b, int blocks, size_t lds) {
                                                 ST_COUNT * b->cap);
   batch_kernel<kMixed><<<blocks, 256, lds, b->stream>>>(b->d_items, b->d_hdr, b->d_counters,
                                                          b->d_stats);
+  if (kMixed)
+    batch_deferred_kernel<<<1024, 256, 0, b->stream>>>(b->d_items, b->d_hdr, b->d_counters,
+                                                       b->d_stats);
   hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
   if (e != hipSuccess) return hip_check(e, "hipStreamEndCapture");
   SDTFE_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -320,12 +399,30 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
     it.first_tile = tiles;
     tiles += nt;
   }
+  const bool mixed = prec == SDTFE_PREC_MIXED;
+  if (mixed) {
+    // room for every pixel of the launch: a batch accumulates, so its first
+    // pass cannot be repeated
+    size_t pixels = 0;
+    for (int i = 0; i < n; ++i)
+      pixels += (size_t)b->h_items[i].n_tiles * b->h_items[i].tile_w * b->h_items[i].tile_h;
+    if (pixels > (size_t)INT_MAX) return fail(SDTFE_EINVAL, "batch too large for the mixed walk");
+    if (pixels > b->defer_cap) {
+      if (b->d_defer) (void)hipFree(b->d_defer);
+      b->d_defer = nullptr;
+      b->defer_cap = 0;
+      const size_t cap = std::max<size_t>(pixels, 65536);
+      if (hipMalloc((void **)&b->d_defer, sizeof(Deferred) * cap) != hipSuccess)
+        return fail(SDTFE_ENOMEM, "deferred-pixel queue allocation failed");
+      b->defer_cap = cap;
+    }
+  }
   BatchHdr &h = *b->h_hdr;
+  h.dq = DeferQueue{b->d_defer, b->d_counters + kXcd, (int)std::min<size_t>(b->defer_cap, INT_MAX)};
   h.n_items = n;
   h.total_tiles = tiles;
   for (int x = 0; x <= kXcd; ++x) h.share[x] = (int)((int64_t)tiles * x / kXcd);
 
-  const bool mixed = prec == SDTFE_PREC_MIXED;
   int &blocks = mixed ? b->blocks_mixed : b->blocks_fp64;
   if (!blocks) blocks = mixed ? blocks_for<true>(b, kLdsBytes) : blocks_for<false>(b, kLdsBytes);
   if (blocks <= 0) return fail(SDTFE_EHIP, "occupancy query failed");
@@ -387,7 +484,7 @@ int sdtfe_batch_create(sdtfe_batch **out) {
   if (!rc) rc = hip_check(hipEventCreate(&b->ev0), "event");
   if (!rc) rc = hip_check(hipEventCreate(&b->ev1), "event");
   if (!rc && (hipMalloc((void **)&b->d_hdr, sizeof(BatchHdr)) != hipSuccess ||
-              hipMalloc((void **)&b->d_counters, sizeof(int) * kXcd) != hipSuccess ||
+              hipMalloc((void **)&b->d_counters, sizeof(int) * (kXcd + 1)) != hipSuccess ||
               hipHostMalloc((void **)&b->h_hdr, sizeof(BatchHdr), hipHostMallocDefault) !=
                   hipSuccess))
     rc = fail(SDTFE_ENOMEM, "batch header");
@@ -408,6 +505,7 @@ void sdtfe_batch_destroy(sdtfe_batch *b) {
   if (b->d_stats) (void)hipFree(b->d_stats);
   if (b->d_hdr) (void)hipFree(b->d_hdr);
   if (b->d_counters) (void)hipFree(b->d_counters);
+  if (b->d_defer) (void)hipFree(b->d_defer);
   if (b->h_items) (void)hipHostFree(b->h_items);
   if (b->h_hdr) (void)hipHostFree(b->h_hdr);
   if (b->ev0) (void)hipEventDestroy(b->ev0);

@@ -30,7 +30,7 @@ void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
 void launch_trace_fp64(sdtfe_tess *t, const MapGeom &g, int n, const double *x, const double *y,
                        const int *entry, const uint64_t *pixel, const uint32_t *sample,
                        double *value, int *crossed);
-void launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate);
+int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate);
 void launch_trace_mixed(sdtfe_tess *t, const MapGeom &g, int n, const double *x,
                         const double *y, const int *entry, const uint64_t *pixel,
                         const uint32_t *sample, double *value, int *crossed);
@@ -141,12 +141,15 @@ void free_tess(sdtfe_tess *t) {
   if (!t) return;
   void *bufs[] = {t->pos4,    t->tets,     t->vol,     t->nabla,   t->rec64,
                   t->rec32,   t->tri_xy,   t->tri_tet, t->tri_abc, t->cell_off,
-                  t->cell_tri, t->stats,   t->map_buf, t->navg_buf};
+                  t->cell_tri, t->stats,   t->map_buf, t->navg_buf, t->defer_buf,
+                  t->defer_count};
   if (t->stream) (void)hipStreamSynchronize(t->stream);
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (t->ev0) (void)hipEventDestroy(t->ev0);
   if (t->ev1) (void)hipEventDestroy(t->ev1);
+  if (t->evk0) (void)hipEventDestroy(t->evk0);
+  if (t->evk1) (void)hipEventDestroy(t->evk1);
   if (t->stream) (void)hipStreamDestroy(t->stream);
   delete t;
 }
@@ -169,6 +172,8 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
   int rc = hip_check(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking), "stream");
   if (!rc) rc = hip_check(hipEventCreate(&t->ev0), "event");
   if (!rc) rc = hip_check(hipEventCreate(&t->ev1), "event");
+  if (!rc) rc = hip_check(hipEventCreate(&t->evk0), "event");
+  if (!rc) rc = hip_check(hipEventCreate(&t->evk1), "event");
   if (!rc && hipMalloc((void **)&t->stats, sizeof(unsigned long long) * ST_COUNT) != hipSuccess)
     rc = fail(SDTFE_ENOMEM, "stats allocation failed");
   if (!rc) rc = build_field(t, pd, tetra);
@@ -183,14 +188,16 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
 }
 
 int run_map(sdtfe_tess *t, const sdtfe_map_params *p, double *out, int *navg, int accumulate,
-            double *t_walk_ms) {
+            double *t_walk_ms, double *t_kernel_ms) {
   const MapGeom g = make_geom_pub(t, *p);
   SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
   SDTFE_HIP(hipEventRecord(t->ev0, t->stream));
-  if (p->precision == SDTFE_PREC_MIXED)
-    launch_map_mixed(t, g, out, navg, accumulate);
-  else
+  if (p->precision == SDTFE_PREC_MIXED) {
+    const int rc = launch_map_mixed(t, g, out, navg, accumulate);
+    if (rc) return rc;
+  } else {
     launch_map_fp64(t, g, out, navg, accumulate);
+  }
   SDTFE_HIP(hipGetLastError());
   SDTFE_HIP(hipEventRecord(t->ev1, t->stream));
   if (t_walk_ms) {
@@ -198,6 +205,8 @@ int run_map(sdtfe_tess *t, const sdtfe_map_params *p, double *out, int *navg, in
     float ms = 0;
     SDTFE_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
     *t_walk_ms = ms;
+    SDTFE_HIP(hipEventElapsedTime(&ms, t->evk0, t->evk1));
+    *t_kernel_ms = ms;
   }
   return 0;
 }
@@ -362,8 +371,8 @@ int sdtfe_map(sdtfe_tess *t, const sdtfe_map_params *p, double *rho, int *n_avg,
   auto t0 = Clock::now();
   const size_t npix = (size_t)p->grid_dim * (size_t)p->grid_dim;
   if ((rc = ensure_map_buf(t, npix))) return rc;
-  double t_walk = 0;
-  if ((rc = run_map(t, p, t->map_buf, t->navg_buf, 0, &t_walk))) return rc;
+  double t_walk = 0, t_kernel = 0;
+  if ((rc = run_map(t, p, t->map_buf, t->navg_buf, 0, &t_walk, &t_kernel))) return rc;
   std::vector<double> val(npix);
   SDTFE_HIP(hipMemcpyAsync(val.data(), t->map_buf, sizeof(double) * npix, hipMemcpyDeviceToHost,
                            t->stream));
@@ -375,6 +384,7 @@ int sdtfe_map(sdtfe_tess *t, const sdtfe_map_params *p, double *rho, int *n_avg,
   if ((rc = read_stats(t, stats))) return rc;
   if (stats) {
     stats->t_walk_ms = t_walk;
+    stats->t_kernel_ms = t_kernel;
     stats->t_total_ms = ms_since(t0);
   }
   return 0;
@@ -387,11 +397,13 @@ int sdtfe_map_device(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_devic
   if (rc) return rc;
   if (!rho_device) return fail(SDTFE_EINVAL, "null rho");
   auto t0 = Clock::now();
-  double t_walk = 0;
-  if ((rc = run_map(t, p, rho_device, nullptr, 1, stats ? &t_walk : nullptr))) return rc;
+  double t_walk = 0, t_kernel = 0;
+  if ((rc = run_map(t, p, rho_device, nullptr, 1, stats ? &t_walk : nullptr, &t_kernel)))
+    return rc;
   if (stats) {
     if ((rc = read_stats(t, stats))) return rc;
     stats->t_walk_ms = t_walk;
+    stats->t_kernel_ms = t_kernel;
     stats->t_total_ms = ms_since(t0);
   }
   return 0;

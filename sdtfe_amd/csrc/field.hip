@@ -445,6 +445,8 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra) {
     t->org[k] = n > 0 ? 0.5 * (lo[k] + hi[k]) : 0.0;
     if (n > 0) t->span = std::max(t->span, 0.5 * (hi[k] - lo[k]));
   }
+  t->z_min = n > 0 ? lo[2] : 0.0;
+  t->z_max = n > 0 ? hi[2] : 0.0;
 
   SDTFE_RC(dev_alloc(t, &t->nabla, 3 * (size_t)nt));
   SDTFE_RC(dev_alloc(t, &t->rec64, (size_t)nt));

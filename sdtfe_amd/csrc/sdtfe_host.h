@@ -43,6 +43,7 @@ struct sdtfe_tess {
   bool given_field = false;  // vertex values taken from particle_data[:,3]
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evk0 = nullptr, evk1 = nullptr;  // around the walk kernel proper
   // particles and tets as uploaded, plus the field
   double *pos4 = nullptr;     // N x 4 (x, y, z, rho)
   int *tets = nullptr;        // T x 8
@@ -52,6 +53,7 @@ struct sdtfe_tess {
   sdtfe::Rec32 *rec32 = nullptr;
   double org[3] = {0, 0, 0};  // centre of the particles' bounding box
   double span = 0;            // its half-extent (max over axes)
+  double z_min = 0, z_max = 0;  // the particles' (and the hull's) z range
   // entry structure
   double *tri_xy = nullptr;
   int *tri_tet = nullptr;
@@ -64,6 +66,9 @@ struct sdtfe_tess {
   double *map_buf = nullptr;            // G*G pixel values
   int *navg_buf = nullptr;
   size_t map_cap = 0;
+  sdtfe::Deferred *defer_buf = nullptr;  // pixels queued for the FP64 pass
+  size_t defer_cap = 0;
+  int *defer_count = nullptr;
   int64_t device_bytes = 0;
   double t_setup_ms = 0;
 };

@@ -42,6 +42,29 @@ struct alignas(16) Rec32 {
 };
 static_assert(sizeof(Rec32) == 96, "Rec32 layout");
 
+constexpr int kDeferWindow = 256;  // handed-over samples one entry can list
+
+// A pixel whose mixed walk handed sample-rays to the FP64 walk (64 B); the
+// second pass of a mixed map finishes it.
+struct alignas(16) Deferred {
+  double sum;         // over the finished samples below `tail`, in sample order
+  uint64_t pix;       // pixel index in its map
+  uint64_t mask[4];   // bit k: sample base + k is walked in FP64
+  int base;           // first handed-over sample
+  int tail;           // samples tail .. samples-1 are all walked in FP64
+  int n_ok;           // finished samples that crossed a tet
+  int item;           // batch item (0 for a single map)
+};
+static_assert(sizeof(Deferred) == 64, "Deferred layout");
+
+// Queue of a launch: count may exceed cap, which the host reads as overflow
+// (the launch is then repeated with a larger queue).
+struct DeferQueue {
+  Deferred *q;
+  int *count;
+  int cap;
+};
+
 // Entry lookup: forward-hull triangles binned on a uniform 2D grid.
 struct EntryGrid {
   const double *tri_xy;   // F x 6: a.x a.y b.x b.y c.x c.y

@@ -1,0 +1,107 @@
+// walk_defer.h -- pixels whose mixed walk handed sample-rays to the FP64
+// walk.
+//
+// The mixed walk returns a ray it cannot take to the FP64 walk with the
+// same tets and draws (walk_mixed's false return).  Walking that ray inside
+// the mixed kernel would pull the FP64 walk's registers into every wave, so
+// the pixel is queued instead: the first pass stores the sum over the samples
+// it finished (in sample order) and which samples it handed over; a second
+// pass walks those in FP64, adds them in sample order and writes the pixel.
+// Every pixel thus has one writer and a fixed summation order, so results do
+// not depend on scheduling.
+#pragma once
+
+#include "walk_mixed_core.h"
+
+namespace sdtfe {
+
+// Per-pixel bookkeeping of the first pass; the window of handed-over samples
+// is 64 kWords wide (1 for a lane that walks its pixel's samples in order and
+// can stop at a tail, 4 when the samples were spread over several lanes).
+template <int kWords>
+struct PixelSamples {
+  double sum = 0.0;
+  uint64_t mask[kWords] = {};
+  int n_ok = 0;
+  int base = -1;
+  int tail;
+
+  SDTFE_HD explicit PixelSamples(int samples) : tail(samples) {}
+
+  // A finished sample (crossed == 0: a miss or a failed walk, which adds
+  // nothing and does not count, src/dtfe.c:626-636).
+  SDTFE_HD void add(double v, int crossed) {
+    sum += v;
+    n_ok += crossed > 0;
+  }
+
+  // A handed-over sample.  Returns false when it falls outside the window:
+  // it and every later sample then go to the FP64 walk (the caller stops).
+  SDTFE_HD bool hand_over(int s) {
+    if (base < 0) base = s;
+    const int k = s - base;
+    if (k >= 64 * kWords) {
+      tail = s;
+      return false;
+    }
+    const uint64_t bit = 1ull << (k & 63);
+    // bit-masked word select: the word index is lane-varying
+    for (int j = 0; j < kWords; ++j) mask[j] |= bit & (0ull - (uint64_t)((k >> 6) == j));
+    return true;
+  }
+
+  SDTFE_HD bool clean(int samples) const { return base < 0 && tail == samples; }
+};
+
+#if defined(__HIPCC__)
+template <int kWords>
+__device__ inline void push_deferred(const DeferQueue &dq, const PixelSamples<kWords> &ps,
+                                     uint64_t pix, int item) {
+  const int i = atomicAdd(dq.count, 1);
+  if (i >= dq.cap) return;  // overflow: the host repeats the launch
+  Deferred d;
+  d.sum = ps.sum;
+  d.pix = pix;
+  for (int k = 0; k < 4; ++k) d.mask[k] = k < kWords ? ps.mask[k] : 0ull;
+  d.base = ps.base;
+  d.tail = ps.tail;
+  d.n_ok = ps.n_ok;
+  d.item = item;
+  dq.q[i] = d;
+}
+#endif
+
+// Second pass for one entry: the handed-over samples and the tail through
+// the FP64 walk, added in sample order.  Returns the pixel's sample sum;
+// n_ok counts the samples that crossed a tet.
+SDTFE_HD inline double finish_deferred(const Deferred *__restrict__ d,
+                                       const Rec64 *__restrict__ rec64, const EntryGrid &eg,
+                                       const MapGeom &g, int &n_ok, LaneCounters &cnt) {
+  const uint64_t pix = d->pix;
+  const int base = d->base, tail = d->tail;
+  const int px = (int)(pix % (uint64_t)g.grid), py = (int)(pix / (uint64_t)g.grid);
+  double sum = d->sum;
+  n_ok = d->n_ok;
+  for (int s = base < 0 ? tail : base; s < g.samples; ++s) {
+    const int k = s - base;
+    if (s < tail && !((d->mask[k >> 6] >> (k & 63)) & 1ull)) continue;
+    const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
+    double xt, yt;
+    sample_point(g, px, py, key, xt, yt);
+    if (s > tail) cnt.samples++;  // the first pass counted the others
+    const int entry = locate_entry(eg, xt, yt);
+    if (entry < 0) {  // only tail samples can miss
+      cnt.missed++;
+      continue;
+    }
+    int crossed;
+    sum += walk64(rec64, entry, xt, yt, g, key, crossed, cnt);
+    cnt.fp64_redo++;
+    cnt.tet_steps += (unsigned)crossed;
+    if (crossed == 0) cnt.failed++;
+    n_ok += crossed > 0;
+  }
+  return sum;
+}
+
+}  // namespace sdtfe

@@ -84,8 +84,10 @@ __global__ void locate_kernel(EntryGrid eg, int n, const double *x, const double
 
 void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
   const int nb = (g.grid + 15) / 16;
+  (void)hipEventRecord(t->evk0, t->stream);
   map_fp64_kernel<<<nb * nb, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg, accumulate,
                                                   t->stats);
+  (void)hipEventRecord(t->evk1, t->stream);
 }
 
 void launch_trace_fp64(sdtfe_tess *t, const MapGeom &g, int n, const double *x, const double *y,

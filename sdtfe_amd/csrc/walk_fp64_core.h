@@ -58,26 +58,31 @@ struct Hit {
 // 7.2, gfx950) miscompiles an if/else (or ?:) chain that picks
 // floating-point operands by a lane-varying face index: lanes on the last arm
 // lose that arm's assignments.  Every such choice here computes all
-// candidates in straight-line code and picks with bit masks.
+// candidates in straight-line code and picks with bit masks: two bit-select
+// levels (v_bfi_b32) on the index's two bits.  f must be in 0..3.
+SDTFE_HD inline uint32_t bsel32(uint32_t m, uint32_t x, uint32_t y) { return (x & m) | (y & ~m); }
+
+SDTFE_HD inline uint32_t pick4u(int f, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+  const uint32_t m0 = 0u - (uint32_t)(f & 1), m1 = 0u - (uint32_t)((f >> 1) & 1);
+  return bsel32(m1, bsel32(m0, d, c), bsel32(m0, b, a));
+}
+
 SDTFE_HD inline double pick4(int f, double a, double b, double c, double d) {
-  const uint64_t m0 = 0ull - (uint64_t)(f == 0), m1 = 0ull - (uint64_t)(f == 1);
-  const uint64_t m2 = 0ull - (uint64_t)(f == 2), m3 = 0ull - (uint64_t)(f == 3);
-  const uint64_t v = (__builtin_bit_cast(uint64_t, a) & m0) | (__builtin_bit_cast(uint64_t, b) & m1) |
-                     (__builtin_bit_cast(uint64_t, c) & m2) | (__builtin_bit_cast(uint64_t, d) & m3);
-  return __builtin_bit_cast(double, v);
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  const uint64_t uc = __builtin_bit_cast(uint64_t, c), ud = __builtin_bit_cast(uint64_t, d);
+  const uint32_t lo = pick4u(f, (uint32_t)ua, (uint32_t)ub, (uint32_t)uc, (uint32_t)ud);
+  const uint32_t hi = pick4u(f, (uint32_t)(ua >> 32), (uint32_t)(ub >> 32), (uint32_t)(uc >> 32),
+                             (uint32_t)(ud >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 SDTFE_HD inline float pick4f(int f, float a, float b, float c, float d) {
-  const uint32_t m0 = 0u - (uint32_t)(f == 0), m1 = 0u - (uint32_t)(f == 1);
-  const uint32_t m2 = 0u - (uint32_t)(f == 2), m3 = 0u - (uint32_t)(f == 3);
-  const uint32_t v = (__builtin_bit_cast(uint32_t, a) & m0) | (__builtin_bit_cast(uint32_t, b) & m1) |
-                     (__builtin_bit_cast(uint32_t, c) & m2) | (__builtin_bit_cast(uint32_t, d) & m3);
-  return __builtin_bit_cast(float, v);
+  return __builtin_bit_cast(float, pick4u(f, __builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
+                                          __builtin_bit_cast(uint32_t, c), __builtin_bit_cast(uint32_t, d)));
 }
 
 SDTFE_HD inline int pick4i(int f, int a, int b, int c, int d) {
-  const int m0 = -(int)(f == 0), m1 = -(int)(f == 1), m2 = -(int)(f == 2), m3 = -(int)(f == 3);
-  return (a & m0) | (b & m1) | (c & m2) | (d & m3);
+  return (int)pick4u(f, (uint32_t)a, (uint32_t)b, (uint32_t)c, (uint32_t)d);
 }
 
 // Crossing heights of the four faces: barycentric weights from each face's
@@ -106,25 +111,16 @@ SDTFE_HD inline FaceHeights face_heights(double uCB, double uAC, double uBA, dou
   return h;
 }
 
-// rayTetraPlucker_full (src/dtfe.c:28-215): faces are tested in the order
-// BDC, ACD, ADB, ABC; a face with any nonzero side sign is an enter face if
-// all its signed sides are >= 0 and no enter face was found yet, else a leave
-// face if all are <= 0; the search stops at the second face found; a zero
-// side on a found face marks a special case.
-SDTFE_HD inline Hit plucker64(const Tet64 &r, double lz, double rx, double ry) {
-  const double p2 = lz, p3 = -lz * ry, p4 = lz * rx;
-  const double uCB = edge_u(p2, p3, p4, r.cx, r.cy, r.bx, r.by);
-  const double uAC = edge_u(p2, p3, p4, r.cx, r.cy, r.ax, r.ay);
-  const double uBA = edge_u(p2, p3, p4, r.bx, r.by, r.ax, r.ay);
-  const double uDC = edge_u(p2, p3, p4, r.cx, r.cy, r.dx, r.dy);
-  const double uBD = edge_u(p2, p3, p4, r.bx, r.by, r.dx, r.dy);
-  const double uDA = edge_u(p2, p3, p4, r.dx, r.dy, r.ax, r.ay);
-  const int sCB = sign_zero(uCB), sAC = sign_zero(uAC), sBA = sign_zero(uBA);
-  const int sDC = sign_zero(uDC), sBD = sign_zero(uBD), sDA = sign_zero(uDA);
+// Face choice of rayTetraPlucker_full (src/dtfe.c:64-213) from the six side
+// signs: faces are tested in the order BDC, ACD, ADB, ABC; a face with any
+// nonzero side sign is an enter face if all its signed sides are >= 0 and
+// no enter face was found yet, else a leave face if all are <= 0; the search
+// stops at the second face found; a zero side on a found face marks a
+// special case.
+SDTFE_HD inline void choose_faces(int sCB, int sAC, int sBA, int sDC, int sBD, int sDA, Hit &h) {
   const int fa[4] = {-sCB, sAC, sDA, sBA};
   const int fb[4] = {sDC, -sDC, sBD, sCB};
   const int fc[4] = {-sBD, -sDA, -sBA, -sAC};
-  Hit h;
   h.enter = -1;
   h.leave = -1;
   h.special = false;
@@ -141,6 +137,21 @@ SDTFE_HD inline Hit plucker64(const Tet64 &r, double lz, double rx, double ry) {
     found += (en || le) ? 1 : 0;
   }
   h.ok = found == 2;
+}
+
+// rayTetraPlucker_full (src/dtfe.c:28-215): side signs, face choice and the
+// two crossing heights.
+SDTFE_HD inline Hit plucker64(const Tet64 &r, double lz, double rx, double ry) {
+  const double p2 = lz, p3 = -lz * ry, p4 = lz * rx;
+  const double uCB = edge_u(p2, p3, p4, r.cx, r.cy, r.bx, r.by);
+  const double uAC = edge_u(p2, p3, p4, r.cx, r.cy, r.ax, r.ay);
+  const double uBA = edge_u(p2, p3, p4, r.bx, r.by, r.ax, r.ay);
+  const double uDC = edge_u(p2, p3, p4, r.cx, r.cy, r.dx, r.dy);
+  const double uBD = edge_u(p2, p3, p4, r.bx, r.by, r.dx, r.dy);
+  const double uDA = edge_u(p2, p3, p4, r.dx, r.dy, r.ax, r.ay);
+  Hit h;
+  choose_faces(sign_zero(uCB), sign_zero(uAC), sign_zero(uBA), sign_zero(uDC), sign_zero(uBD),
+               sign_zero(uDA), h);
   const FaceHeights zf = face_heights(uCB, uAC, uBA, uDC, uBD, uDA, r);
   h.z_in = pick4(h.enter, zf.z[0], zf.z[1], zf.z[2], zf.z[3]);
   h.z_out = pick4(h.leave, zf.z[0], zf.z[1], zf.z[2], zf.z[3]);

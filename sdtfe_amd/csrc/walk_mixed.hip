@@ -1,21 +1,34 @@
 // walk_mixed.hip -- map and trace kernels of the mixed-precision walk
-// (walk_mixed_core.h).
-#include "walk_mixed_core.h"
+// (walk_mixed_core.h).  A map is two passes: the mixed walk over every
+// pixel, then the FP64 walk over the queued pixels' handed-over samples
+// (walk_defer.h).
+#include "walk_defer.h"
 
 namespace sdtfe {
 
+// LDS copy of the face_code table; every thread of the block calls this.
+__device__ inline void load_faces(uint8_t *faces) {
+  if (threadIdx.x < kFaceCodes) faces[threadIdx.x] = (uint8_t)face_code((int)threadIdx.x);
+  __syncthreads();
+}
+
+__global__ void defer_reset_kernel(int *count) { *count = 0; }
+
+template <bool kFullSlab>
 __global__ __launch_bounds__(256) void map_mixed_kernel(const Rec32 *__restrict__ rec32,
                                                        const Rec64 *__restrict__ rec64,
                                                        EntryGrid eg, MapGeom g, MixedConsts mc,
                                                        double *out, int *navg, int accumulate,
+                                                       DeferQueue dq,
                                                        unsigned long long *stats) {
+  __shared__ uint8_t faces[kFaceCodes];
+  load_faces(faces);
   int px, py;
   lane_pixel(g.grid, px, py);
   LaneCounters cnt;
   if (px < g.grid && py < g.grid) {
     const uint64_t pix = (uint64_t)py * (uint64_t)g.grid + (uint64_t)px;
-    double sum = 0.0;
-    int n_avg = g.samples;
+    PixelSamples<1> ps(g.samples);
     for (int s = 0; s < g.samples; ++s) {
       const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
       double xt, yt;
@@ -24,41 +37,82 @@ __global__ __launch_bounds__(256) void map_mixed_kernel(const Rec32 *__restrict_
       const int entry = locate_entry(eg, xt, yt);
       if (entry < 0) {
         cnt.missed++;
-        --n_avg;
         continue;
       }
+      double v;
       int crossed;
-      const double v = walk_mixed(rec32, rec64, entry, xt, yt, g, mc.flt, key, crossed, cnt);
-      sum += v;
-      cnt.tet_steps += (unsigned)crossed;
-      if (crossed == 0) {
-        cnt.failed++;
-        --n_avg;
+      if (walk_mixed<kFullSlab>(rec32, rec64, faces, entry, xt, yt, g, mc.flt, v, crossed,
+                                cnt)) {
+        ps.add(v, crossed);
+        cnt.tet_steps += (unsigned)crossed;
+        if (crossed == 0) cnt.failed++;
+      } else if (!ps.hand_over(s)) {
+        break;
       }
     }
-    const double val = n_avg > 0 ? sum / n_avg : 0.0;
-    if (accumulate)
-      out[pix] += val;
-    else
-      out[pix] = val;
-    if (navg) navg[pix] = n_avg;
+    if (ps.clean(g.samples)) {
+      const double val = ps.n_ok > 0 ? ps.sum / ps.n_ok : 0.0;
+      if (accumulate)
+        out[pix] += val;
+      else
+        out[pix] = val;
+      if (navg) navg[pix] = ps.n_ok;
+    } else {
+      push_deferred(dq, ps, pix, 0);
+    }
   }
   cnt.flush(stats);
 }
 
+// Second pass: grid-stride over the queued pixels.
+__global__ __launch_bounds__(256) void map_deferred_kernel(const Rec64 *__restrict__ rec64,
+                                                          EntryGrid eg, MapGeom g, double *out,
+                                                          int *navg, int accumulate,
+                                                          DeferQueue dq,
+                                                          unsigned long long *stats) {
+  LaneCounters cnt;
+  const int n = min(*dq.count, dq.cap);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int n_ok;
+    const double sum = finish_deferred(dq.q + i, rec64, eg, g, n_ok, cnt);
+    const uint64_t pix = dq.q[i].pix;
+    const double val = n_ok > 0 ? sum / n_ok : 0.0;
+    if (accumulate)
+      out[pix] += val;
+    else
+      out[pix] = val;
+    if (navg) navg[pix] = n_ok;
+  }
+  cnt.flush(stats);
+}
+
+// Rays one by one; a handed-over ray is walked in FP64 in place (this
+// kernel serves inspection, not maps).
 __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
                                    const Rec64 *__restrict__ rec64, EntryGrid eg, MapGeom g,
                                    MixedConsts mc, int n, const double *x, const double *y,
                                    const int *entry_in, const uint64_t *pixel,
                                    const uint32_t *sample, double *value, int *crossed_out,
                                    unsigned long long *stats) {
+  __shared__ uint8_t faces[kFaceCodes];
+  load_faces(faces);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   LaneCounters cnt;
   if (i < n) {
     const uint64_t key = sample_key(g.seed, pixel[i], sample[i]);
     const int entry = entry_in ? entry_in[i] : locate_entry(eg, x[i], y[i]);
     int crossed;
-    value[i] = walk_mixed(rec32, rec64, entry, x[i], y[i], g, mc.flt, key, crossed, cnt);
+    double v;
+    const bool done = mc.flt.full_slab
+                          ? walk_mixed<true>(rec32, rec64, faces, entry, x[i], y[i], g, mc.flt,
+                                             v, crossed, cnt)
+                          : walk_mixed<false>(rec32, rec64, faces, entry, x[i], y[i], g, mc.flt,
+                                              v, crossed, cnt);
+    if (!done) {
+      cnt.fp64_redo++;
+      v = walk64(rec64, entry, x[i], y[i], g, key, crossed, cnt);
+    }
+    value[i] = v;
     crossed_out[i] = crossed;
     cnt.tet_steps += (unsigned)crossed;
   }
@@ -67,11 +121,58 @@ __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
 
 extern const bool kHaveMixed = true;
 
-void launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
+namespace {
+
+int ensure_defer(sdtfe_tess *t, size_t entries) {
+  if (!t->defer_count && hipMalloc((void **)&t->defer_count, sizeof(int)) != hipSuccess)
+    return fail(SDTFE_ENOMEM, "deferred-pixel counter allocation failed");
+  if (entries <= t->defer_cap) return 0;
+  if (t->defer_buf) (void)hipFree(t->defer_buf);
+  t->defer_buf = nullptr;
+  t->defer_cap = 0;
+  if (hipMalloc((void **)&t->defer_buf, sizeof(Deferred) * entries) != hipSuccess)
+    return fail(SDTFE_ENOMEM, "deferred-pixel queue allocation failed");
+  t->defer_cap = entries;
+  return 0;
+}
+
+}  // namespace
+
+// The queue holds a quarter of the pixels; a map that overflows it is run
+// again with room for all of them (an accumulating map gets that room from
+// the start, since its first pass cannot be repeated).
+int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
+  const size_t npix = (size_t)g.grid * (size_t)g.grid;
+  if (npix > (size_t)INT_MAX) return fail(SDTFE_EINVAL, "map too large for the mixed walk");
+  const size_t want = accumulate ? npix : std::max(npix / 4, std::min<size_t>(npix, 65536));
+  int rc = ensure_defer(t, want);
+  if (rc) return rc;
+  const MixedConsts mc = mixed_consts(t, g);
   const int nb = (g.grid + 15) / 16;
-  map_mixed_kernel<<<nb * nb, 256, 0, t->stream>>>(t->rec32, t->rec64, t->grid, g,
-                                                   mixed_consts(t, g), out, navg, accumulate,
-                                                   t->stats);
+  for (;;) {
+    const DeferQueue dq{t->defer_buf, t->defer_count, (int)std::min(t->defer_cap, npix)};
+    defer_reset_kernel<<<1, 1, 0, t->stream>>>(t->defer_count);
+    SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
+    if (mc.flt.full_slab)
+      map_mixed_kernel<true><<<nb * nb, 256, 0, t->stream>>>(
+          t->rec32, t->rec64, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+    else
+      map_mixed_kernel<false><<<nb * nb, 256, 0, t->stream>>>(
+          t->rec32, t->rec64, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+    SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
+    map_deferred_kernel<<<1024, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg,
+                                                     accumulate, dq, t->stats);
+    SDTFE_HIP(hipGetLastError());
+    int count = 0;
+    SDTFE_HIP(hipMemcpyAsync(&count, t->defer_count, sizeof(int), hipMemcpyDeviceToHost,
+                             t->stream));
+    SDTFE_HIP(hipStreamSynchronize(t->stream));
+    if (count <= dq.cap) return 0;
+    // overflow: the queued pixels beyond the queue were not written; every
+    // other pixel is written again by the repeat
+    SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
+    if ((rc = ensure_defer(t, npix))) return rc;
+  }
 }
 
 void launch_trace_mixed(sdtfe_tess *t, const MapGeom &g, int n, const double *x,

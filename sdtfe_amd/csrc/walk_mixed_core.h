@@ -1,35 +1,37 @@
 // walk_mixed_core.h -- the mixed-precision ray walk (gfx950).
 //
-// Same tets, same faces, same perturbations as the FP64 walk; less FP64
-// arithmetic and a smaller record (sdtfe::Rec32, 96 B: neighbours, vertex A
-// in FP64, B - A, C - A, D - A and the four vertex densities in FP32):
+// Same tets, same faces, same perturbations as the FP64 walk; far fewer
+// instructions per tet and a smaller record (sdtfe::Rec32, 96 B: neighbours,
+// vertex A in FP64, B - A, C - A, D - A and the four vertex densities in
+// FP32).  The walk is issue-bound, and on CDNA4 an unpacked FP32 operation
+// costs what an FP64 one does, so the gain comes from doing less per tet:
 //
 //  * Local coordinates.  The ray's offset from A is formed in FP64 and
 //    rounded to FP32; the other vertices' offsets add the stored edges.  The
-//    FP32 rounding error is therefore a fraction of the local tet size, not
-//    of the box, so tiny tets in halo cores lose nothing.
-//  * Side tests in FP32 with a filter.  The Plücker product of the ray with
-//    edge (X, Y) is lz * orient2d(p; X, Y) (src/dtfe.c:24-26 expanded) and is
-//    formed as (X - p) x (Y - p).  An a-priori bound covers the rounding of
-//    the offsets and of the products; the FP32 sign is taken only when |c|
-//    exceeds that bound plus the reference's zero band (ZERO = 1e4
-//    DBL_EPSILON, src/constants.h:6, over lz) plus the FP64 expression's own
-//    error.  Otherwise the product is redone in FP64 with the reference
-//    expression and Sign().  Every side sign is therefore the FP64 walk's
-//    sign, so the walk visits the same tets.
-//  * A tet whose face choice is special (a zero side) or fails is handed to
-//    the FP64 core, which perturbs the ray exactly as the FP64 walk does.
-//  * Each face crossing is computed once (as the leave face, height and
-//    density relative to A in FP32, height made absolute in FP64) and
-//    carried as the next tet's entry; errors in a crossing then cancel to
-//    first order because the DTFE field is continuous across the face.  The
-//    field is linear along the chord, so the trapezoid rule on the two
-//    crossings is its exact integral; this never forms
-//    rho_A + grad.(p - A), which is ill-conditioned for sliver tets.
-//  * The ray's first and last crossings (their errors have no neighbour to
-//    cancel against), crossings with ill-conditioned FP32 weights (nearly
-//    vertical faces) and midpoints at the slab boundary are done in FP64.  Per-ray and per-pixel
-//    sums stay FP64; volumes and vertex densities are the FP64 ones.
+//    FP32 rounding error is therefore a fraction of the local tet size.
+//  * Side tests in FP32 with one filter per tet.  The Plücker product of the
+//    ray with edge (X, Y) is lz * orient2d(p; X, Y) (src/dtfe.c:24-26
+//    expanded), formed as (X - p) x (Y - p).  With L the largest L1 norm of
+//    the four offsets, every side's error is below k_err L^2; a side whose
+//    magnitude exceeds that plus the reference's zero band (ZERO =
+//    1e4 DBL_EPSILON, src/constants.h:6, over lz) plus the FP64 expression's
+//    own error has the FP64 walk's sign and is not zero there.
+//  * Face choice by table.  With six nonzero signs the enter and leave faces
+//    follow from the 6-bit sign mask (face_code); the table sits in LDS.
+//  * Only the leave face's crossing is formed (height and density relative
+//    to A, FP32 barycentric weights); it is carried as the next tet's entry,
+//    so errors cancel to first order (the DTFE field is continuous across the
+//    face).  The field is linear along the chord, so the trapezoid rule on
+//    the two crossings is its exact integral.
+//  * One FP64 block handles everything rare: a tet with an undecided side,
+//    a failed face choice, an ill-conditioned leave face (nearly vertical),
+//    a midpoint at the slab boundary, the first tet of a ray and a tet after
+//    a turn-around.  It runs the FP64 walk's own step (perturbation
+//    included) and its crossings.  The ray's last crossing (it has no
+//    neighbour to cancel against) is redone in FP64 after the loop, where
+//    the lanes of a wave are together again.
+//  Per-ray and per-pixel sums stay FP64; volumes and vertex densities are the
+//  FP64 ones.
 #pragma once
 
 #include <algorithm>
@@ -38,9 +40,6 @@
 #include "walk_fp64_core.h"
 
 namespace sdtfe {
-
-// Largest tolerated relative uncertainty of FP32 barycentric weights.
-constexpr float kCondMax = 1e-4f;
 
 struct Tet32 {
   int nb0, nb1, nb2, nb3;
@@ -67,45 +66,38 @@ SDTFE_HD inline void load_tet32(const Rec32 *__restrict__ rec, int t, Tet32 &r) 
 
 SDTFE_HD inline int nb32(const Tet32 &r, int f) { return pick4i(f, r.nb0, r.nb1, r.nb2, r.nb3); }
 
-// Per-launch constants of the filter (host-computed, mixed_consts()).
-struct Filter {
-  float k_prod;  // relative rounding of the two products and their difference
-  float k_off;   // rounding of an offset, relative to the L1 norms it is built from
-  float thr;     // (ZERO + FP64 expression error) / lz
-};
+// Face choice when no side is zero.  Bit e of w is set when side e is
+// negative, sides in the order CB, AC, BA, DC, BD, DA.  Face f (BDC, ACD,
+// ADB, ABC) uses the sides in kFaceSides[f]; it is an enter face when
+// exactly the sides in kFaceEnterNeg[f] are negative among them and a leave
+// face when exactly the others are.  The first enter and the first leave
+// face in face order are taken, as src/dtfe.c:64-213 takes them (with no
+// zero side a face is of one kind or neither, and the search only stops
+// once both are found).  Code: enter in bits 0-1, leave in bits 2-3, bit 4
+// set when both exist.
+constexpr int kFaceSides[4] = {0x19, 0x2A, 0x34, 0x07};
+constexpr int kFaceEnterNeg[4] = {0x11, 0x28, 0x04, 0x02};
+constexpr int kFaceCodes = 64;
 
-// One side product: FP32 value, its error bound and its sign (0: undecided).
-struct Side {
-  float c;
-  float b;
-  int s;
-};
-
-// (x1, y1) = X - p, (x2, y2) = Y - p in FP32; l1 = |x| + |y| of each and
-// e1, e2 their rounding bounds (per component).
-SDTFE_HD inline Side side32(float x1, float y1, float l1, float e1, float x2, float y2, float l2,
-                            float e2, const Filter &f) {
-  const float p1 = x1 * y2, p2 = y1 * x2;
-  const float c = p1 - p2;
-  const float b = f.k_prod * (fabsf(p1) + fabsf(p2)) + e1 * l2 + e2 * l1 + e1 * e2 + f.thr;
-  Side s;
-  s.c = c;
-  s.b = b;
-  s.s = c > b ? 1 : (c < -b ? -1 : 0);
-  return s;
+SDTFE_HD constexpr int face_code(int w) {
+  int en = -1, le = -1;
+  for (int f = 0; f < 4; ++f) {
+    const int s = w & kFaceSides[f];
+    if (en < 0 && s == kFaceEnterNeg[f])
+      en = f;
+    else if (le < 0 && s == (kFaceSides[f] ^ kFaceEnterNeg[f]))
+      le = f;
+  }
+  return en >= 0 && le >= 0 ? (en | le << 2 | 16) : 0;
 }
 
-struct Cross {
-  double z;  // absolute height
-  float rho;
-};
-
-// Crossing points of the four faces from FP32 side values: height and
-// density relative to A, each the barycentric mix of the face's three
-// vertices with the weights the FP64 walk uses for the height (face_z);
-// straight-line for all faces, callers pick one (see pick4).
-struct Crosses {
-  float z[4], rho[4], inv[4];  // inv: 1 / (sum of the face's signed sides)
+// Per-launch constants of the filter (host-computed, mixed_consts()).
+struct Filter {
+  float k_err;    // side error bound over L^2
+  float thr;      // (ZERO + FP64 expression error) / lz
+  float k_ill;    // a face whose weight sum is below k_ill L^2 is ill-conditioned
+  float k_band;   // crossing height uncertainty over the tet's height range
+  int full_slab;  // the slab holds the hull's z range: every midpoint is inside
 };
 
 // Reciprocal to within 1 ulp (v_rcp_f32) on the device.
@@ -117,253 +109,241 @@ SDTFE_HD inline float rcp32(float x) {
 #endif
 }
 
-SDTFE_HD inline void face_point32(float t0, float t1, float t2, float w1n, float w2n, float z0,
-                                  float z1, float z2, float q0, float q1, float q2, float &z,
-                                  float &q, float &inv) {
-  inv = rcp32(t0 + t1 + t2);
-  const float w1 = w1n * inv, w2 = w2n * inv;
-  z = fmaf(w1, z1 - z0, fmaf(w2, z2 - z0, z0));
-  q = fmaf(w1, q1 - q0, fmaf(w2, q2 - q0, q0));
+// Crossing of face f from the FP32 side values: height (relative to A) and
+// density, each the barycentric mix of the face's three vertices with the
+// weights the FP64 walk uses for the height (face_z, src/dtfe.c:67-74 and
+// its three siblings).  Per face the weight sum and the weighted differences
+// from the face's first vertex are formed, then picked.
+struct Sides32 {
+  float cCB, cAC, cBA, cDC, cBD, cDA;
+};
+
+struct Cross32 {
+  float z, rho, den;
+};
+
+SDTFE_HD inline Cross32 face_cross32(int f, const Sides32 &c, const Tet32 &r) {
+  const float zb = r.ebz, zc = r.ecz, zd = r.edz;
+  const float den0 = (-c.cCB + c.cDC) - c.cBD, den1 = (c.cAC - c.cDC) - c.cDA;
+  const float den2 = (c.cDA + c.cBD) - c.cBA, den3 = (c.cBA + c.cCB) - c.cAC;
+  const float nz0 = fmaf(-c.cBD, zc - zd, c.cDC * (zb - zd));
+  const float nz1 = fmaf(c.cAC, zd - zc, c.cDC * zc);
+  const float nz2 = fmaf(c.cBD, -zb, -c.cBA * (zd - zb));
+  const float nz3 = fmaf(-c.cAC, zb, c.cBA * zc);
+  const float nr0 = fmaf(-c.cBD, r.rc - r.rd, c.cDC * (r.rb - r.rd));
+  const float nr1 = fmaf(c.cAC, r.rd - r.rc, -c.cDC * (r.ra - r.rc));
+  const float nr2 = fmaf(c.cBD, r.ra - r.rb, -c.cBA * (r.rd - r.rb));
+  const float nr3 = fmaf(-c.cAC, r.rb - r.ra, c.cBA * (r.rc - r.ra));
+  Cross32 x;
+  x.den = pick4f(f, den0, den1, den2, den3);
+  const float inv = rcp32(x.den);
+  x.z = fmaf(pick4f(f, nz0, nz1, nz2, nz3), inv, pick4f(f, zd, zc, zb, 0.f));
+  x.rho = fmaf(pick4f(f, nr0, nr1, nr2, nr3), inv, pick4f(f, r.rd, r.rc, r.rb, r.ra));
+  return x;
 }
 
-SDTFE_HD inline Crosses face_crosses32(float cCB, float cAC, float cBA, float cDC, float cBD,
-                                       float cDA, const Tet32 &r) {
-  Crosses c;
-  const float za = 0.f, zb = r.ebz, zc = r.ecz, zd = r.edz;
-  // BDC: D + w1 (C - D) + w2 (B - D); ACD: C, D, A; ADB: B, A, D; ABC: A, B, C
-  face_point32(-cCB, cDC, -cBD, -cBD, cDC, zd, zc, zb, r.rd, r.rc, r.rb, c.z[0], c.rho[0],
-               c.inv[0]);
-  face_point32(cAC, -cDC, -cDA, cAC, -cDC, zc, zd, za, r.rc, r.rd, r.ra, c.z[1], c.rho[1],
-               c.inv[1]);
-  face_point32(cDA, cBD, -cBA, cBD, -cBA, zb, za, zd, r.rb, r.ra, r.rd, c.z[2], c.rho[2],
-               c.inv[2]);
-  face_point32(cBA, cCB, -cAC, -cAC, cBA, za, zb, zc, r.ra, r.rb, r.rc, c.z[3], c.rho[3],
-               c.inv[3]);
+// The same crossing in FP64 from the FP64 record, for the ray's first and
+// last crossings (their errors have no neighbour to cancel against; hull
+// tets are large).  Face f's vertices (P0, P1, P2) are (f^3, f^2, f^1) in
+// the order the reference mixes them; with s(X, Y) = (X - p) x (Y - p) the
+// weights of P1 and P2 are s(P0, P2) and s(P1, P0) over the sum of the three
+// edges' s (src/dtfe.c:67-74 and siblings, lz cancelled).  Loads only the
+// three vertices.
+struct Cross64 {
+  double z;
+  float rho;
+  double cond;  // sum of the weights' magnitudes over their sum's
+};
+
+SDTFE_HD inline Cross64 face_cross64(const Rec64 *__restrict__ rec, int t, int f, double x,
+                                     double y, const Tet32 &r) {
+  const double *v = reinterpret_cast<const double *>(reinterpret_cast<const char *>(rec + t) + 16);
+  const int i0 = f ^ 3, i1 = f ^ 2, i2 = f ^ 1;
+  const double x0 = v[3 * i0] - x, y0 = v[3 * i0 + 1] - y, z0 = v[3 * i0 + 2];
+  const double x1 = v[3 * i1] - x, y1 = v[3 * i1 + 1] - y, z1 = v[3 * i1 + 2];
+  const double x2 = v[3 * i2] - x, y2 = v[3 * i2 + 1] - y, z2 = v[3 * i2 + 2];
+  const double s0 = x2 * y1 - y2 * x1, s1 = x0 * y2 - y0 * x2, s2 = x1 * y0 - y1 * x0;
+  const double sum = s0 + s1 + s2;
+  const double inv = 1.0 / sum;
+  const double w1 = s1 * inv, w2 = s2 * inv;
+  const double r0 = pick4f(i0, r.ra, r.rb, r.rc, r.rd), r1 = pick4f(i1, r.ra, r.rb, r.rc, r.rd);
+  const double r2 = pick4f(i2, r.ra, r.rb, r.rc, r.rd);
+  Cross64 c;
+  c.z = z0 + w1 * (z1 - z0) + w2 * (z2 - z0);
+  c.rho = (float)(r0 + w1 * (r1 - r0) + w2 * (r2 - r0));
+  c.cond = (fabs(s0) + fabs(s1) + fabs(s2)) / fabs(sum);
   return c;
 }
 
-// c ? a : b per lane, without control flow (see pick4).
-SDTFE_HD inline Cross sel_cross(bool c, const Cross &a, const Cross &b) {
-  const int f = c ? 0 : 1;
-  return Cross{pick4(f, a.z, b.z, b.z, b.z), pick4f(f, a.rho, b.rho, b.rho, b.rho)};
-}
-
-SDTFE_HD inline Cross pick_cross(int f, const Crosses &c, double az) {
-  return Cross{az + (double)pick4f(f, c.z[0], c.z[1], c.z[2], c.z[3]),
-               pick4f(f, c.rho[0], c.rho[1], c.rho[2], c.rho[3])};
-}
-
-// The same crossing from the FP64 record: side products with the reference
-// expression, weights and height in FP64; the density mixes the FP32 vertex
-// densities.
-SDTFE_HD inline Cross face_cross64(int f, const Tet64 &q, double lz, double ox, double oy,
-                                   const Tet32 &r) {
-  const double p2 = lz, p3 = -lz * oy, p4 = lz * ox;
-  const double uCB = edge_u(p2, p3, p4, q.cx, q.cy, q.bx, q.by);
-  const double uAC = edge_u(p2, p3, p4, q.cx, q.cy, q.ax, q.ay);
-  const double uBA = edge_u(p2, p3, p4, q.bx, q.by, q.ax, q.ay);
-  const double uDC = edge_u(p2, p3, p4, q.cx, q.cy, q.dx, q.dy);
-  const double uBD = edge_u(p2, p3, p4, q.bx, q.by, q.dx, q.dy);
-  const double uDA = edge_u(p2, p3, p4, q.dx, q.dy, q.ax, q.ay);
-  const double ra = r.ra, rb = r.rb, rc = r.rc, rd = r.rd;
-  const double t[4][5] = {{-uCB, uDC, -uBD, -uBD, uDC},
-                          {uAC, -uDC, -uDA, uAC, -uDC},
-                          {uDA, uBD, -uBA, uBD, -uBA},
-                          {uBA, uCB, -uAC, -uAC, uBA}};
-  const double zv[4][3] = {{q.dz, q.cz, q.bz}, {q.cz, q.dz, q.az}, {q.bz, q.az, q.dz},
-                           {q.az, q.bz, q.cz}};
-  const double rv[4][3] = {{rd, rc, rb}, {rc, rd, ra}, {rb, ra, rd}, {ra, rb, rc}};
-  double zz[4], rr[4];
+// The FP64 walk's signs for the sides whose FP32 value is within the
+// filter's bound: the reference expression (src/dtfe.c:24-26) on the FP64
+// record and its zero band.  Updates the sign mask w; false when a side is
+// zero there (the FP64 walk then perturbs the ray).
+SDTFE_HD inline bool sides64(const Rec64 *__restrict__ rec, int t, double x, double y, double lz,
+                             const Sides32 &c, float bound, uint32_t &w) {
+  const double *v = reinterpret_cast<const double *>(reinterpret_cast<const char *>(rec + t) + 16);
+  const double p2 = lz, p3 = -lz * y, p4 = lz * x;
+  const float cs[6] = {c.cCB, c.cAC, c.cBA, c.cDC, c.cBD, c.cDA};
+  constexpr int kX[6] = {2, 2, 1, 2, 1, 3}, kY[6] = {1, 0, 0, 3, 3, 0};  // CB AC BA DC BD DA
+  bool ok = true;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const double inv = 1.0 / (t[k][0] + t[k][1] + t[k][2]);
-    const double w1 = t[k][3] * inv, w2 = t[k][4] * inv;
-    zz[k] = zv[k][0] + w1 * (zv[k][1] - zv[k][0]) + w2 * (zv[k][2] - zv[k][0]);
-    rr[k] = rv[k][0] + w1 * (rv[k][1] - rv[k][0]) + w2 * (rv[k][2] - rv[k][0]);
+  for (int e = 0; e < 6; ++e) {
+    if (!(fabsf(cs[e]) > bound)) {
+      const double u = edge_u(p2, p3, p4, v[3 * kX[e]], v[3 * kX[e] + 1], v[3 * kY[e]],
+                              v[3 * kY[e] + 1]);
+      const int sg = sign_zero(u);
+      ok = ok && sg != 0;
+      w = (w & ~(1u << e)) | ((uint32_t)(sg < 0) << e);
+    }
   }
-  return Cross{pick4(f, zz[0], zz[1], zz[2], zz[3]), (float)pick4(f, rr[0], rr[1], rr[2], rr[3])};
+  return ok;
 }
 
-// Relative uncertainty of face f's FP32 barycentric weights: the summed
-// bounds of its sides over the magnitude of their signed sum.
-SDTFE_HD inline float face_cond(int f, const Side &CB, const Side &AC, const Side &BA,
-                                const Side &DC, const Side &BD, const Side &DA,
-                                const Crosses &cr) {
-  const float b0 = CB.b + DC.b + BD.b, b1 = AC.b + DC.b + DA.b;
-  const float b2 = DA.b + BD.b + BA.b, b3 = BA.b + CB.b + AC.b;
-  return pick4f(f, b0 * fabsf(cr.inv[0]), b1 * fabsf(cr.inv[1]), b2 * fabsf(cr.inv[2]),
-                b3 * fabsf(cr.inv[3]));
-}
-
-// Face choice from the six side signs (src/dtfe.c:64-213, as plucker64).
-SDTFE_HD inline void choose_faces(int sCB, int sAC, int sBA, int sDC, int sBD, int sDA,
-                                  int &enter, int &leave, bool &ok, bool &special) {
-  const int fa[4] = {-sCB, sAC, sDA, sBA};
-  const int fb[4] = {sDC, -sDC, sBD, sCB};
-  const int fc[4] = {-sBD, -sDA, -sBA, -sAC};
-  enter = -1;
-  leave = -1;
-  special = false;
-  int found = 0;
-#pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    const int a = fa[f], b = fb[f], c = fc[f];
-    const bool active = found < 2 && (a | b | c) != 0;
-    const bool en = active && enter == -1 && a >= 0 && b >= 0 && c >= 0;
-    const bool le = active && !en && leave == -1 && a <= 0 && b <= 0 && c <= 0;
-    if (en) enter = f;
-    if (le) leave = f;
-    if ((en || le) && (a == 0 || b == 0 || c == 0)) special = true;
-    found += (en || le) ? 1 : 0;
-  }
-  ok = found == 2;
-}
-
-// One sample-ray.  (x, y) FP64 sample position; the walk keeps the FP64
-// position, which only a perturbation changes.
-SDTFE_HD inline double walk_mixed(const Rec32 *__restrict__ rec32,
-                                  const Rec64 *__restrict__ rec64, int entry, double x,
-                                  double y, const MapGeom &g, const Filter &flt, uint64_t key,
-                                  int &crossed, LaneCounters &cnt) {
-  crossed = 0;
-  if (entry < 0) return 0.0;
-  const double step = (g.dxy / 2.0) / kPerturbMax;
+// Whether the chord from height z_in to z_out through tet t (faces enter,
+// leave) counts: its midpoint strictly inside the slab, as the FP64 walk
+// decides it (src/dtfe.c:356-369).  The FP32 crossing heights are within
+// k_band of the tet's height range of the FP64 walk's; a midpoint that
+// close to a slab bound is decided again from both crossings in FP64, with a
+// band for their own rounding (a few ulps times the weights' conditioning).
+// False when even that leaves it open (the FP64 walk then takes the ray).
+SDTFE_HD inline bool slab_inside(const Rec64 *__restrict__ rec64, int t, int enter, int leave,
+                                 double x, double y, const Tet32 &r, double z_in, double z_out,
+                                 const MapGeom &g, const Filter &flt, bool &inside) {
   const double z_lo = g.off_z, z_hi = g.off_z + g.lz;
-  double ox = x, oy = y, acc = 0.0;
-  uint32_t draw = 2;
-  int cur = entry, prev = -1, f_in = -1, f_out = 0, nxt = entry, steps = 0;
-  Cross carry{0.0, 0.f};
-  bool have_carry = false;
+  const double zm = z_in + 0.5 * (z_out - z_in);
+  inside = zm > z_lo && zm < z_hi;
+  const float zext = fmaxf(fmaxf(fabsf(r.ebz), fabsf(r.ecz)), fabsf(r.edz));
+  const double band = (double)(flt.k_band * zext) + 1e-12 * g.lz;
+  if (!(fabs(zm - z_lo) < band || fabs(zm - z_hi) < band)) return true;
+  // rare: the midpoint from FP64 crossings
+  const Cross64 ci = face_cross64(rec64, t, enter, x, y, r);
+  const Cross64 co = face_cross64(rec64, t, leave, x, y, r);
+  const double zm64 = ci.z + 0.5 * (co.z - ci.z);
+  const double band64 = 1e-12 * g.lz + 1e-14 * (double)zext * (ci.cond + co.cond);
+  if (fabs(zm64 - z_lo) < band64 || fabs(zm64 - z_hi) < band64) return false;
+  inside = zm64 > z_lo && zm64 < z_hi;
+  return true;
+}
+
+// One sample-ray.  (x, y) FP64 sample position.  Returns false when the ray
+// must be walked by the FP64 walk instead (walk64 with the same key): an
+// undecided side (a possible zero, hence a possible perturbation), a failed
+// face choice, a turn-around, an ill-conditioned crossing or a midpoint at
+// the slab boundary.  Otherwise value and crossed are the ray's integral and
+// tet count, and the tets are the FP64 walk's.  faces: the kFaceCodes
+// face_code values (LDS on the device).  kFullSlab: the slab holds the
+// hull's z range (Filter::full_slab), no midpoint test.
+template <bool kFullSlab>
+SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec64,
+                                const uint8_t *faces, int entry, double x, double y,
+                                const MapGeom &g, const Filter &flt, double &value, int &crossed,
+                                LaneCounters &cnt) {
+  value = 0.0;
+  crossed = 0;
+  if (entry < 0) return true;
+  const double z_lo = g.off_z, z_hi = g.off_z + g.lz;
+  double acc = 0.0;
+  int cur = entry, prev = -1, f_in = 0, f_out = 0, steps = 0;
+  double zc = 0.0;  // carried crossing: absolute height
+  float rc = 0.f;   // and density
   Tet32 r;
+  // why the loop ended: 0 the hull was left (last crossing pending), 1 the
+  // FP64 walk takes the ray, 2 the step cap
+  int end = 0;
   for (;;) {
-    if (steps > 0) nxt = nb32(r, f_out);
-    if (nxt == -1) break;
-    if (nxt == prev) {  // turn around (src/dtfe.c:303-309)
-      const int tmp = f_in;
-      f_in = f_out;
-      f_out = tmp;
-      nxt = nb32(r, f_out);
-      if (nxt == -1) break;
-      have_carry = false;
+    const int nxt = steps > 0 ? nb32(r, f_out) : entry;
+    // a turn-around (src/dtfe.c:303-309) follows a perturbation only
+    if (nxt == prev) {
+      end = 1;
+      break;
     }
     if (++steps > g.max_steps) {
-      cnt.step_cap++;
-      crossed = 0;
-      return 0.0;
+      end = 2;
+      break;
     }
     prev = cur;
     cur = nxt;
     load_tet32(rec32, cur, r);
 
     // offsets of the vertices from the ray, FP32, relative to A
-    const float ax = (float)(r.ax - ox), ay = (float)(r.ay - oy);
+    const float ax = (float)(r.ax - x), ay = (float)(r.ay - y);
     const float bx = ax + r.ebx, by = ay + r.eby, cx = ax + r.ecx, cy = ay + r.ecy;
     const float dx = ax + r.edx, dy = ay + r.edy;
-    const float la = fabsf(ax) + fabsf(ay), lb = fabsf(bx) + fabsf(by);
-    const float lc = fabsf(cx) + fabsf(cy), ld = fabsf(dx) + fabsf(dy);
-    // rounding bounds per component: A - p once, the others also carry the
-    // stored edge and the sum
-    const float ea = flt.k_off * la;
-    const float eb = flt.k_off * (la + fabsf(r.ebx) + fabsf(r.eby) + lb);
-    const float ec = flt.k_off * (la + fabsf(r.ecx) + fabsf(r.ecy) + lc);
-    const float ed = flt.k_off * (la + fabsf(r.edx) + fabsf(r.edy) + ld);
-    Side CB = side32(cx, cy, lc, ec, bx, by, lb, eb, flt);
-    Side AC = side32(cx, cy, lc, ec, ax, ay, la, ea, flt);
-    Side BA = side32(bx, by, lb, eb, ax, ay, la, ea, flt);
-    Side DC = side32(cx, cy, lc, ec, dx, dy, ld, ed, flt);
-    Side BD = side32(bx, by, lb, eb, dx, dy, ld, ed, flt);
-    Side DA = side32(dx, dy, ld, ed, ax, ay, la, ea, flt);
-    const bool undecided =
-        (CB.s == 0) | (AC.s == 0) | (BA.s == 0) | (DC.s == 0) | (BD.s == 0) | (DA.s == 0);
-    if (undecided) {
-      // undecided sides: the reference expression in FP64
-      Tet64 q;
-      load_tet(rec64, cur, q);
-      const double p2 = g.lz, p3 = -g.lz * oy, p4 = g.lz * ox;
-      const float il = (float)(1.0 / g.lz);
-      unsigned redo = 0;
-#define SDTFE_REDO(S, XX, XY, YX, YY)                                    \
-  if (S.s == 0) {                                                        \
-    const double u = edge_u(p2, p3, p4, q.XX, q.XY, q.YX, q.YY);         \
-    S.s = sign_zero(u);                                                  \
-    S.c = (float)u * il;                                                 \
-    S.b = 0.f;                                                           \
-    ++redo;                                                              \
-  }
-      SDTFE_REDO(CB, cx, cy, bx, by)
-      SDTFE_REDO(AC, cx, cy, ax, ay)
-      SDTFE_REDO(BA, bx, by, ax, ay)
-      SDTFE_REDO(DC, cx, cy, dx, dy)
-      SDTFE_REDO(BD, bx, by, dx, dy)
-      SDTFE_REDO(DA, dx, dy, ax, ay)
-#undef SDTFE_REDO
-      cnt.fp64_redo += redo;
-    }
-    int enter, leave;
-    bool ok, special;
-    choose_faces(CB.s, AC.s, BA.s, DC.s, BD.s, DA.s, enter, leave, ok, special);
-    if (!ok || special) {
-      // the FP64 walk's own step, perturbation included
-      Tet64 q;
-      load_tet(rec64, cur, q);
-      Hit h = plucker64(q, g.lz, ox, oy);
-      if (!h.ok) {
-        cnt.no_isect++;
-        crossed = 0;
-        return 0.0;
+    const float L = fmaxf(fmaxf(fabsf(ax) + fabsf(ay), fabsf(bx) + fabsf(by)),
+                          fmaxf(fabsf(cx) + fabsf(cy), fabsf(dx) + fabsf(dy)));
+    const float L2 = L * L;
+    Sides32 c;
+    c.cCB = fmaf(cx, by, -(cy * bx));
+    c.cAC = fmaf(cx, ay, -(cy * ax));
+    c.cBA = fmaf(bx, ay, -(by * ax));
+    c.cDC = fmaf(cx, dy, -(cy * dx));
+    c.cBD = fmaf(bx, dy, -(by * dx));
+    c.cDA = fmaf(dx, ay, -(dy * ax));
+    const float cmin = fminf(fminf(fminf(fabsf(c.cCB), fabsf(c.cAC)), fabsf(c.cBA)),
+                             fminf(fminf(fabsf(c.cDC), fabsf(c.cBD)), fabsf(c.cDA)));
+    const float bound = fmaf(flt.k_err, L2, flt.thr);
+    uint32_t w = (__builtin_bit_cast(uint32_t, c.cCB) >> 31) |
+                 ((__builtin_bit_cast(uint32_t, c.cAC) >> 31) << 1) |
+                 ((__builtin_bit_cast(uint32_t, c.cBA) >> 31) << 2) |
+                 ((__builtin_bit_cast(uint32_t, c.cDC) >> 31) << 3) |
+                 ((__builtin_bit_cast(uint32_t, c.cBD) >> 31) << 4) |
+                 ((__builtin_bit_cast(uint32_t, c.cDA) >> 31) << 5);
+    if (!(cmin > bound)) {
+      // rare: the sides the filter leaves open get the FP64 walk's sign; a
+      // zero there means a perturbation, which the FP64 walk does
+      if (!sides64(rec64, cur, x, y, g.lz, c, bound, w)) {
+        end = 1;
+        break;
       }
-      if (h.special && !resolve_special(q, g.lz, step, key, draw, ox, oy, h, cnt)) {
-        crossed = 0;
-        return 0.0;
-      }
-      f_in = h.enter;
-      f_out = h.leave;
-      ++crossed;
-      acc += tet_integral64(q, ox, oy, h.z_in, h.z_out, g.off_z, g.lz);
-      have_carry = false;  // the next tet starts from its own entry face
-      continue;
     }
+    const int code = faces[w];
+    if (!(code & 16)) {
+      end = 1;
+      break;
+    }
+    const int enter = code & 3, leave = (code >> 2) & 3;
     f_in = enter;
+    if (steps == 1) {  // first tet: the entry crossing in FP64
+      const Cross64 in = face_cross64(rec64, cur, enter, x, y, r);
+      zc = in.z;
+      rc = in.rho;
+    }
     f_out = leave;
     ++crossed;
-    const Crosses cr = face_crosses32(CB.c, AC.c, BA.c, DC.c, BD.c, DA.c, r);
-    const Cross fresh = pick_cross(enter, cr, r.az);
-    Cross in = sel_cross(have_carry, carry, fresh);
-    Cross out = pick_cross(leave, cr, r.az);
-    // FP64 for the crossings whose error no neighbour cancels (where the ray
-    // enters and leaves the hull) and for ill-conditioned weights
-    const bool in64 = !have_carry;
-    const bool out64 =
-        nb32(r, leave) == -1 || face_cond(leave, CB, AC, BA, DC, BD, DA, cr) > kCondMax;
-    if (in64 | out64) {
-      Tet64 q;
-      load_tet(rec64, cur, q);
-      const Cross in_64 = face_cross64(enter, q, g.lz, ox, oy, r);
-      const Cross out_64 = face_cross64(leave, q, g.lz, ox, oy, r);
-      in = sel_cross(in64, in_64, in);
-      out = sel_cross(out64, out_64, out);
-      cnt.fp64_redo++;
+    if (nb32(r, leave) == -1) break;  // leaves the hull: last crossing below
+    Cross32 out = face_cross32(leave, c, r);
+    double z_out = r.az + (double)out.z;
+    if (fabsf(out.den) < flt.k_ill * L2) {
+      // rare: FP32 weights of a nearly vertical face are too uncertain
+      const Cross64 o = face_cross64(rec64, cur, leave, x, y, r);
+      z_out = o.z;
+      out.rho = o.rho;
     }
-    carry = out;
-    have_carry = true;
-    const double dz = out.z - in.z;
-    const double zm = in.z + 0.5 * dz;
-    bool inside = zm > z_lo && zm < z_hi;
-    const double band = 1e-4 * fabs(dz) + 1e-12 * g.lz;
-    if (fabs(zm - z_lo) < band || fabs(zm - z_hi) < band) {
-      // at the slab boundary: the FP64 walk's own midpoint decides
-      Tet64 q;
-      load_tet(rec64, cur, q);
-      const Hit h = plucker64(q, g.lz, ox, oy);
-      const double dz64 = h.z_out - h.z_in;
-      const double zm64 = dz64 / 2 + h.z_in;
-      inside = zm64 > z_lo && zm64 < z_hi;
-      cnt.fp64_redo++;
+    const double dz = z_out - zc;
+    bool inside = true;
+    if (!kFullSlab && !slab_inside(rec64, cur, enter, leave, x, y, r, zc, z_out, g, flt, inside)) {
+      end = 1;
+      break;
     }
-    // the field is linear along the chord: trapezoid = exact integral
-    if (inside) acc += (double)(0.5f * (in.rho + out.rho)) * dz;
+    if (inside) acc += (double)(0.5f * (rc + out.rho)) * dz;
+    zc = z_out;
+    rc = out.rho;
   }
-  return acc;
+  if (end == 1) return false;
+  if (end == 2) {
+    cnt.step_cap++;
+    crossed = 0;
+    return true;
+  }
+  // the last crossing in FP64, where the lanes of a wave are together again
+  const Cross64 out = face_cross64(rec64, cur, f_out, x, y, r);
+  const double dz = out.z - zc;
+  bool inside = true;
+  if (!kFullSlab && !slab_inside(rec64, cur, f_in, f_out, x, y, r, zc, out.z, g, flt, inside))
+    return false;
+  if (inside) acc += (double)(0.5f * (rc + out.rho)) * dz;
+  value = acc;
+  return true;
 }
 
 struct MixedConsts {
@@ -377,15 +357,26 @@ inline MixedConsts mixed_consts(const sdtfe_tess *t, const MapGeom &g) {
   const double amax = std::max(t->span + std::max(std::fabs(t->org[0]), std::fabs(t->org[1])),
                                g.ray_span + std::max(std::fabs(g.org_x), std::fabs(g.org_y)));
   MixedConsts m;
-  m.flt.k_prod = (float)(3.0 * u32);
-  // each offset is a rounded FP64 difference, a rounded stored edge and a
-  // rounded sum: within u32 of the L1 norms involved, padded for the
-  // rounding of the bound itself
-  m.flt.k_off = (float)(1.01 * u32);
+  // offsets: A - p rounds twice (u32 + 2 u64), a stored edge once more and
+  // the sum once: every component is within 4 u32 L.  A side's product and
+  // fma add 2 u32 L^2, the offset errors 2 (4 u32 L) L; 11 pads the bound's
+  // own rounding.
+  m.flt.k_err = (float)(11.0 * u32);
   // reference zero band and the FP64 expression's own rounding (three
   // products and two sums of terms below 2 lz amax^2), over lz
   const double err64 = 16.0 * std::ldexp(1.0, -53) * 2.0 * g.lz * amax * amax;
   m.flt.thr = (float)(1.01 * (1e4 * 2.220446049250313080847e-16 + err64) / g.lz);
+  // weight sums below 1e-3 L^2 (projected face area under 0.05 % of L^2)
+  // leave the weights too uncertain; such rays take the FP64 walk
+  m.flt.k_ill = 1e-3f;
+  // a weight is within 4 k_err / k_ill (one side's error over the sum, and
+  // the sum's error times the weight) and a face's height range is at most
+  // twice the tet's: crossing heights are within 8 k_err / k_ill of the
+  // tet's height range, plus the FP32 rounding of the mix
+  m.flt.k_band = (float)(8.1 * m.flt.k_err / m.flt.k_ill + 1e-6);
+  // every chord midpoint lies within the hull's z range: a midpoint on the
+  // boundary belongs to a chord of zero length, which adds nothing
+  m.flt.full_slab = t->n > 0 && g.off_z <= t->z_min && g.off_z + g.lz >= t->z_max;
   return m;
 }
 

@@ -6,9 +6,29 @@
 #include "walk_mixed_core.h"
 
 #include <cstdio>
+#include <string>
 #include <vector>
 
+// The face table against the general face choice on all 64 sign patterns.
+static int check_faces() {
+  int bad = 0;
+  for (int w = 0; w < sdtfe::kFaceCodes; ++w) {
+    int s[6];
+    for (int e = 0; e < 6; ++e) s[e] = (w >> e) & 1 ? -1 : 1;
+    // sides CB, AC, BA, DC, BD, DA through plucker64's face logic
+    sdtfe::Hit h;
+    sdtfe::choose_faces(s[0], s[1], s[2], s[3], s[4], s[5], h);
+    const int want = h.ok ? (h.enter | h.leave << 2 | 16) : 0;
+    if (sdtfe::face_code(w) != want) {
+      fprintf(stderr, "w=%d code %d want %d\n", w, sdtfe::face_code(w), want);
+      ++bad;
+    }
+  }
+  return bad ? 9 : 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc == 2 && std::string(argv[1]) == "faces") return check_faces();
   if (argc != 3 && argc != 4) return 2;
   const bool mixed = argc == 4;
   FILE *f = fopen(argv[1], "rb");
@@ -30,11 +50,11 @@ int main(int argc, char **argv) {
       fread(ent.data(), 8, (size_t)n, f) != (size_t)n)
     return 6;
   std::vector<sdtfe::Rec32> rec32;
-  double org_span[4] = {0, 0, 0, 0};
+  double org_span[6] = {0, 0, 0, 0, 0, 0};  // origin, half-extent, z range
   if (mixed) {
     rec32.resize((size_t)nt);
     if (fread(rec32.data(), sizeof(sdtfe::Rec32), (size_t)nt, f) != (size_t)nt ||
-        fread(org_span, 8, 4, f) != 4)
+        fread(org_span, 8, 6, f) != 6)
       return 8;
   }
   fclose(f);
@@ -52,9 +72,14 @@ int main(int argc, char **argv) {
   tt.org[0] = org_span[0];
   tt.org[1] = org_span[1];
   tt.org[2] = org_span[2];
+  tt.n = mixed ? 1 : 0;
+  tt.z_min = org_span[4];
+  tt.z_max = org_span[5];
   for (long long i = 0; i < n; ++i)
     g.ray_span = std::max(g.ray_span, std::max(std::fabs(x[i] - g.org_x), std::fabs(y[i] - g.org_y)));
   const sdtfe::MixedConsts mc = sdtfe::mixed_consts(&tt, g);
+  uint8_t faces[sdtfe::kFaceCodes];
+  for (int w = 0; w < sdtfe::kFaceCodes; ++w) faces[w] = (uint8_t)sdtfe::face_code(w);
   long long redo = 0;
   FILE *o = fopen(argv[2], "wb");
   if (!o) return 7;
@@ -62,16 +87,24 @@ int main(int argc, char **argv) {
     sdtfe::LaneCounters c;
     int crossed = 0;
     const uint64_t key = sdtfe::sample_key(seed, pix[i], (uint32_t)smp[i]);
-    const double v =
-        mixed ? sdtfe::walk_mixed(rec32.data(), rec.data(), (int)ent[i], x[i], y[i], g, mc.flt,
-                                  key, crossed, c)
-              : sdtfe::walk64(rec.data(), (int)ent[i], x[i], y[i], g, key, crossed, c);
+    double v = 0.0;
+    // the mixed walk hands some rays to the FP64 walk, as the map kernels do
+    const bool done =
+        mixed && (mc.flt.full_slab
+                      ? sdtfe::walk_mixed<true>(rec32.data(), rec.data(), faces, (int)ent[i], x[i],
+                                                y[i], g, mc.flt, v, crossed, c)
+                      : sdtfe::walk_mixed<false>(rec32.data(), rec.data(), faces, (int)ent[i], x[i],
+                                                 y[i], g, mc.flt, v, crossed, c));
+    if (!done) {
+      if (mixed) c.fp64_redo++;
+      v = sdtfe::walk64(rec.data(), (int)ent[i], x[i], y[i], g, key, crossed, c);
+    }
     redo += c.fp64_redo;
     const double cc = crossed;
     fwrite(&v, 8, 1, o);
     fwrite(&cc, 8, 1, o);
   }
   fclose(o);
-  if (mixed) fprintf(stderr, "fp64_redo %lld\n", redo);
+  if (mixed) fprintf(stderr, "fp64_redo %lld full_slab %d\n", redo, mc.flt.full_slab);
   return 0;
 }

@@ -63,3 +63,20 @@ def test_mixed_map_matches_fp64(kind, n, G, S, box_z, cz):
     _say(f"  max rel err {e.max():.2e}, median {np.median(e):.2e}, "
          f"fp64 redo / step {s32['fp64_redo'] / max(1, s32['tet_steps']):.2e}")
     assert e.max() <= RTOL_MIXED
+
+
+def test_mixed_map_is_deterministic():
+    """Two runs of a mixed map agree bit for bit: each pixel has one writer
+    and a fixed summation order, also where samples go to the FP64 pass."""
+    _say("[gpu] mixed map twice")
+    pos = workloads.clustered(131072, seed=9)
+    tetra = triangulate(pos)
+    with capi.Tessellation(to_particle_data(pos), tetra) as t:
+        a, sa, na = t.map(256, 8, want_navg=True, seed=3, precision=capi.PREC_MIXED)
+        b, sb, nb = t.map(256, 8, want_navg=True, seed=3, precision=capi.PREC_MIXED)
+    _say(f"  handed to fp64: {sa['fp64_redo']} of {sa['samples']} samples")
+    assert sa["fp64_redo"] > 0  # the second pass ran
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(na, nb)
+    counts = lambda st: {k: v for k, v in st.items() if not k.startswith("t_")}  # noqa: E731
+    assert counts(sa) == counts(sb)

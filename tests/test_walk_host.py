@@ -68,7 +68,8 @@ def run_walk(host_walk, tmp_path, g, mixed=False):
             lo, hi = g["pos"].min(0), g["pos"].max(0)
             org = 0.5 * (lo + hi)
             records32(g["pos"], g["tetra"], g["rho"]).tofile(f)
-            np.array([*org, float((0.5 * (hi - lo)).max())]).tofile(f)
+            zr = (lo[2], hi[2])
+            np.array([*org, float((0.5 * (hi - lo)).max()), *zr]).tofile(f)
     r = subprocess.run([host_walk, str(inp), str(out)] + (["mixed"] if mixed else []),
                        check=True, capture_output=True, text=True)
     res = np.fromfile(out).reshape(-1, 2)
@@ -82,12 +83,30 @@ def test_device_walk_code_matches_reference(golden, host_walk, tmp_path):
     np.testing.assert_array_equal(res[:, 0], g["value"])
 
 
-def test_device_mixed_walk_code(golden, host_walk, tmp_path):
-    """Mixed walk: same tets as the reference walk, values within 1e-5."""
-    g = golden("g2_rays.npz")
+def test_face_table(host_walk):
+    """The mixed walk's sign-mask face table equals the general face choice
+    (the FP64 walk's, pinned above) on all 64 sign patterns."""
+    r = subprocess.run([host_walk, "faces"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("slab", ["hull", "cut"])
+def test_device_mixed_walk_code(golden, host_walk, tmp_path, slab):
+    """Mixed walk: same tets as the reference walk, values within 1e-5.
+    'hull': the golden rays, whose slab holds the hull's z range (midpoints
+    are not tested); 'cut': the same rays through the slab 0.25 < z < 0.75,
+    against the FP64 walk code, so that chords are cut at both ends."""
+    g = dict(golden("g2_rays.npz"))
+    if slab == "cut":
+        g["lz"], g["off_z"] = 0.5, 0.25
+        want, _ = run_walk(host_walk, tmp_path, g)
+        ref, crossed = want[:, 0], want[:, 1].astype(np.int32)
+    else:
+        ref, crossed = g["value"], g["crossed"]
     res, err = run_walk(host_walk, tmp_path, g, mixed=True)
-    np.testing.assert_array_equal(res[:, 1].astype(np.int32), g["crossed"])
-    v, ref = res[:, 0], g["value"]
+    assert f"full_slab {int(slab == 'hull')}" in err, err
+    np.testing.assert_array_equal(res[:, 1].astype(np.int32), crossed)
+    v = res[:, 0]
     m = ref != 0
     assert np.max(np.abs(v[m] - ref[m]) / np.abs(ref[m])) <= 1e-5
     assert (v[~m] == 0).all()
