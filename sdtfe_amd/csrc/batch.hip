@@ -33,6 +33,7 @@ constexpr int kLdsBytes = 32 * 1024;
 struct BatchItem {
   const Rec64 *rec64;
   const Rec32 *rec32;
+  const FaceRec *facerec;
   EntryGrid grid;
   MapGeom geom;
   MixedConsts mc;
@@ -97,9 +98,9 @@ __device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int
   }
   if (kMixed) {
     const bool done = it.mc.flt.full_slab
-                          ? walk_mixed<true>(it.rec32, it.rec64, faces, entry, xt, yt, g,
+                          ? walk_mixed<true>(it.rec32, it.rec64, it.facerec, faces, entry, xt, yt, g,
                                              it.mc.flt, v, crossed, cnt)
-                          : walk_mixed<false>(it.rec32, it.rec64, faces, entry, xt, yt, g,
+                          : walk_mixed<false>(it.rec32, it.rec64, it.facerec, faces, entry, xt, yt, g,
                                               it.mc.flt, v, crossed, cnt);
     if (!done) return false;
   } else {
@@ -110,14 +111,12 @@ __device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int
   return true;
 }
 
+// The persistent loop; lds: per-sample values for lanes-per-pixel > 1.
 template <bool kMixed>
-__global__ __launch_bounds__(256) void batch_kernel(const BatchItem *__restrict__ items,
-                                                    const BatchHdr *__restrict__ hdr,
-                                                    int *counters,
-                                                    unsigned long long *item_stats) {
-  extern __shared__ double lds[];  // per-sample values for lanes-per-pixel > 1
-  __shared__ int s_tile;
-  __shared__ uint8_t faces[kFaceCodes];
+__device__ __forceinline__ void batch_body(const BatchItem *__restrict__ items,
+                                           const BatchHdr *__restrict__ hdr, int *counters,
+                                           unsigned long long *item_stats, double *lds,
+                                           int &s_tile, uint8_t *faces) {
   if (kMixed && threadIdx.x < kFaceCodes) faces[threadIdx.x] = (uint8_t)face_code((int)threadIdx.x);
   const int home = blockIdx.x % kXcd;
   const int n_items = hdr->n_items;
@@ -208,34 +207,53 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchItem *__restrict_
   }
 }
 
-// Second pass of a mixed batch: grid-stride over the queued pixels of every
-// item; counters go straight to the item's stats (lanes of a wave serve
-// different items).
+__global__ __launch_bounds__(256) void batch_kernel_fp64(const BatchItem *__restrict__ items,
+                                                         const BatchHdr *__restrict__ hdr,
+                                                         int *counters,
+                                                         unsigned long long *item_stats) {
+  extern __shared__ double lds[];
+  __shared__ int s_tile;
+  batch_body<false>(items, hdr, counters, item_stats, lds, s_tile, nullptr);
+}
+
+// Built for 4 waves per SIMD: the persistent loop holds more state than the
+// single-map kernel, and at 5 the allocator spills inside the walk loop.
+#ifndef SDTFE_BATCH_MIXED_WAVES
+#define SDTFE_BATCH_MIXED_WAVES 4
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDTFE_BATCH_MIXED_WAVES))) void
+batch_kernel_mixed(const BatchItem *__restrict__ items, const BatchHdr *__restrict__ hdr,
+                   int *counters, unsigned long long *item_stats) {
+  extern __shared__ double lds[];
+  __shared__ int s_tile;
+  __shared__ uint8_t faces[kFaceCodes];
+  batch_body<true>(items, hdr, counters, item_stats, lds, s_tile, faces);
+}
+
+template <bool kMixed>
+constexpr auto batch_kernel() {
+  if constexpr (kMixed)
+    return batch_kernel_mixed;
+  else
+    return batch_kernel_fp64;
+}
+
+// Second pass of a mixed batch: one wave per queued pixel of any item.
 __global__ __launch_bounds__(256) void batch_deferred_kernel(const BatchItem *__restrict__ items,
                                                             const BatchHdr *__restrict__ hdr,
                                                             const int *counters,
                                                             unsigned long long *item_stats) {
   const DeferQueue dq = hdr->dq;
   const int n = min(counters[kXcd], dq.cap);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const int waves = gridDim.x * (blockDim.x >> 6);
+  for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
     const Deferred *d = dq.q + i;
     const BatchItem &it = items[d->item];
     LaneCounters cnt;
     int n_ok;
-    const double sum = finish_deferred(d, it.rec64, it.grid, it.geom, n_ok, cnt);
-    it.out[d->pix] += n_ok > 0 ? sum / n_ok : 0.0;
-    unsigned long long *st = item_stats + (size_t)d->item * ST_COUNT;
-    if (cnt.samples) atomicAdd(st + ST_SAMPLES, cnt.samples);
-    if (cnt.missed) atomicAdd(st + ST_MISSED, (unsigned long long)cnt.missed);
-    if (cnt.failed) atomicAdd(st + ST_FAILED, (unsigned long long)cnt.failed);
-    if (cnt.no_isect) atomicAdd(st + ST_NO_ISECT, (unsigned long long)cnt.no_isect);
-    if (cnt.pert_exh) atomicAdd(st + ST_PERT_EXHAUSTED, (unsigned long long)cnt.pert_exh);
-    if (cnt.no_isect_pert)
-      atomicAdd(st + ST_NO_ISECT_PERT, (unsigned long long)cnt.no_isect_pert);
-    if (cnt.pert_steps) atomicAdd(st + ST_PERT_STEPS, (unsigned long long)cnt.pert_steps);
-    if (cnt.tet_steps) atomicAdd(st + ST_TET_STEPS, cnt.tet_steps);
-    if (cnt.step_cap) atomicAdd(st + ST_STEP_CAP, (unsigned long long)cnt.step_cap);
-    if (cnt.fp64_redo) atomicAdd(st + ST_FP64_REDO, (unsigned long long)cnt.fp64_redo);
+    const double sum = finish_deferred_wave(d, it.rec64, it.grid, it.geom, n_ok, cnt);
+    if ((threadIdx.x & 63) == 0) it.out[d->pix] += n_ok > 0 ? sum / n_ok : 0.0;
+    cnt.flush(item_stats + (size_t)d->item * ST_COUNT);  // the wave's lanes share the item
   }
 }
 
@@ -298,7 +316,7 @@ int ensure_cap(sdtfe_batch *b, int n) {
 template <bool kMixed>
 int blocks_for(sdtfe_batch *, size_t lds) {
   int per_cu = 0;
-  SDTFE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<kMixed>, 256,
+  SDTFE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<kMixed>(), 256,
                                                          lds));
   int dev = 0;
   SDTFE_HIP(hipGetDevice(&dev));
@@ -314,7 +332,7 @@ int capture(sdtfe_batch *b, int blocks, size_t lds) {
   SDTFE_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   batch_reset_kernel<<<64, 256, 0, b->stream>>>(b->d_hdr, b->d_counters, b->d_stats,
                                                 ST_COUNT * b->cap);
-  batch_kernel<kMixed><<<blocks, 256, lds, b->stream>>>(b->d_items, b->d_hdr, b->d_counters,
+  batch_kernel<kMixed>()<<<blocks, 256, lds, b->stream>>>(b->d_items, b->d_hdr, b->d_counters,
                                                          b->d_stats);
   if (kMixed)
     batch_deferred_kernel<<<1024, 256, 0, b->stream>>>(b->d_items, b->d_hdr, b->d_counters,
@@ -375,6 +393,7 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
     std::memset(&it, 0, sizeof it);
     it.rec64 = tess[i]->rec64;
     it.rec32 = tess[i]->rec32;
+    it.facerec = tess[i]->facerec;
     it.grid = tess[i]->grid;
     it.geom = make_geom_pub(tess[i], p);
     it.mc = mixed_consts(tess[i], it.geom);

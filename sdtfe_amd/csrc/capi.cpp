@@ -140,7 +140,7 @@ struct DevArray {
 void free_tess(sdtfe_tess *t) {
   if (!t) return;
   void *bufs[] = {t->pos4,    t->tets,     t->vol,     t->nabla,   t->rec64,
-                  t->rec32,   t->tri_xy,   t->tri_tet, t->tri_abc, t->cell_off,
+                  t->rec32,   t->facerec,   t->tri_xy,   t->tri_tet, t->tri_abc, t->cell_off,
                   t->cell_tri, t->stats,   t->map_buf, t->navg_buf, t->defer_buf,
                   t->defer_count};
   if (t->stream) (void)hipStreamSynchronize(t->stream);

@@ -247,6 +247,41 @@ __global__ void gradient_record_kernel(const double4 *pos, const int *tets, int 
   rec32[t] = q;
 }
 
+// Face records of the mixed walk (sdtfe::FaceRec), thread per (tet, face).
+__global__ void face_record_kernel(const double4 *pos, const int *tets, int nt, FaceRec *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4 * (int64_t)nt) return;
+  const int t = (int)(i >> 2), f = (int)(i & 3);
+  FaceRec r;
+  r.nb = tets[8 * (int64_t)t + 4 + f];
+  r.code = 0;
+  r.qx = r.qy = 0.0;
+  r.dqz = r.rho = 0.f;
+  if (r.nb >= 0) {
+    const int n = r.nb;
+    int back = -1;
+    for (int k = 3; k >= 0; --k)
+      if (tets[8 * (int64_t)n + 4 + k] == t) back = k;
+    unsigned code = back < 0 ? kFaceRecBad : (unsigned)back;
+    for (int k = 1; k <= 3; ++k) {
+      const int v = tets[8 * (int64_t)t + (f ^ k)];
+      int lab = -1;
+      for (int m = 3; m >= 0; --m)
+        if (tets[8 * (int64_t)n + m] == v) lab = m;
+      if (lab < 0 || lab == back) code |= kFaceRecBad;
+      code |= (unsigned)(lab & 3) << (2 * k);
+    }
+    r.code = code;
+    const double4 q = pos[tets[8 * (int64_t)n + (back & 3)]];
+    const double4 ref = pos[tets[8 * (int64_t)t + (f ^ 1)]];
+    r.qx = q.x;
+    r.qy = q.y;
+    r.dqz = (float)(q.z - ref.z);
+    r.rho = (float)q.w;
+  }
+  out[i] = r;
+}
+
 // ============================================================ entry grid
 
 // Face f of a tet as the triangle the reference emits (src/triangulate.c:
@@ -451,9 +486,13 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra) {
   SDTFE_RC(dev_alloc(t, &t->nabla, 3 * (size_t)nt));
   SDTFE_RC(dev_alloc(t, &t->rec64, (size_t)nt));
   SDTFE_RC(dev_alloc(t, &t->rec32, (size_t)nt));
-  if (nt > 0)
+  SDTFE_RC(dev_alloc(t, &t->facerec, 4 * (size_t)nt));
+  if (nt > 0) {
     gradient_record_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->nabla, t->rec64, t->rec32);
+    face_record_kernel<<<blocks_for(4 * (int64_t)nt, 256), 256, 0, s>>>(
+        reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->facerec);
+  }
   SDTFE_HIP(hipGetLastError());
   SDTFE_HIP(hipStreamSynchronize(s));
   return 0;

@@ -51,6 +51,7 @@ struct sdtfe_tess {
   double *nabla = nullptr;    // T x 3
   sdtfe::Rec64 *rec64 = nullptr;
   sdtfe::Rec32 *rec32 = nullptr;
+  sdtfe::FaceRec *facerec = nullptr;  // 4 per tet
   double org[3] = {0, 0, 0};  // centre of the particles' bounding box
   double span = 0;            // its half-extent (max over axes)
   double z_min = 0, z_max = 0;  // the particles' (and the hull's) z range

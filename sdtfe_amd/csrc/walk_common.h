@@ -58,9 +58,11 @@ SDTFE_HD inline void sample_point(const MapGeom &g, int px, int py, uint64_t key
 }
 
 struct LaneCounters {
-  unsigned missed = 0, failed = 0, no_isect = 0, pert_exh = 0, no_isect_pert = 0;
+  // per lane between flushes (a pixel, or a batch tile): 32 bits hold the
+  // sample counts; tets crossed take 64
+  unsigned samples = 0, missed = 0, failed = 0, no_isect = 0, pert_exh = 0, no_isect_pert = 0;
   unsigned pert_steps = 0, step_cap = 0, fp64_redo = 0;
-  unsigned long long samples = 0, tet_steps = 0;
+  unsigned long long tet_steps = 0;
 
   __device__ void flush(unsigned long long *st) {
     add_stat(st, ST_SAMPLES, samples);

@@ -71,37 +71,78 @@ __device__ inline void push_deferred(const DeferQueue &dq, const PixelSamples<kW
 }
 #endif
 
-// Second pass for one entry: the handed-over samples and the tail through
-// the FP64 walk, added in sample order.  Returns the pixel's sample sum;
-// n_ok counts the samples that crossed a tet.
-SDTFE_HD inline double finish_deferred(const Deferred *__restrict__ d,
-                                       const Rec64 *__restrict__ rec64, const EntryGrid &eg,
-                                       const MapGeom &g, int &n_ok, LaneCounters &cnt) {
+// Samples of an entry: the handed-over ones in order, then tail .. samples-1.
+SDTFE_HD inline int deferred_count(const Deferred *__restrict__ d, int samples) {
+  int n = samples - d->tail;
+  if (d->base >= 0)
+    for (int w = 0; w < 4; ++w) n += __builtin_popcountll(d->mask[w]);
+  return n;
+}
+
+SDTFE_HD inline int deferred_sample(const Deferred *__restrict__ d, int k) {
+  if (d->base >= 0)
+    for (int w = 0; w < 4; ++w) {
+      uint64_t m = d->mask[w];
+      const int c = __builtin_popcountll(m);
+      if (k < c) {
+        for (int i = 0; i < k; ++i) m &= m - 1;  // drop the k lowest set bits
+        return d->base + 64 * w + __builtin_ctzll(m);
+      }
+      k -= c;
+    }
+  return d->tail + k;
+}
+
+// One sample of an entry through the FP64 walk: its value (0 for a miss or
+// a failed walk) and whether it crossed a tet.
+SDTFE_HD inline double deferred_walk(const Deferred *__restrict__ d, int s,
+                                     const Rec64 *__restrict__ rec64, const EntryGrid &eg,
+                                     const MapGeom &g, bool &ok, LaneCounters &cnt) {
   const uint64_t pix = d->pix;
-  const int base = d->base, tail = d->tail;
   const int px = (int)(pix % (uint64_t)g.grid), py = (int)(pix / (uint64_t)g.grid);
+  const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
+  double xt, yt;
+  sample_point(g, px, py, key, xt, yt);
+  if (s > d->tail) cnt.samples++;  // the first pass counted the others
+  ok = false;
+  const int entry = locate_entry(eg, xt, yt);
+  if (entry < 0) {  // only tail samples can miss
+    cnt.missed++;
+    return 0.0;
+  }
+  int crossed;
+  const double v = walk64(rec64, entry, xt, yt, g, key, crossed, cnt);
+  cnt.fp64_redo++;
+  cnt.tet_steps += (unsigned)crossed;
+  if (crossed == 0) cnt.failed++;
+  ok = crossed > 0;
+  return v;
+}
+
+#if defined(__HIPCC__)
+// Second pass for one entry by one wave: lane j walks the entry's samples
+// j, j + 64, ...; the values are added in sample order.  Every lane returns
+// the pixel's sample sum and n_ok (samples that crossed a tet).
+__device__ inline double finish_deferred_wave(const Deferred *__restrict__ d,
+                                              const Rec64 *__restrict__ rec64,
+                                              const EntryGrid &eg, const MapGeom &g, int &n_ok,
+                                              LaneCounters &cnt) {
+  const int lane = threadIdx.x & 63;
+  const int total = deferred_count(d, g.samples);
   double sum = d->sum;
   n_ok = d->n_ok;
-  for (int s = base < 0 ? tail : base; s < g.samples; ++s) {
-    const int k = s - base;
-    if (s < tail && !((d->mask[k >> 6] >> (k & 63)) & 1ull)) continue;
-    const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
-    double xt, yt;
-    sample_point(g, px, py, key, xt, yt);
-    if (s > tail) cnt.samples++;  // the first pass counted the others
-    const int entry = locate_entry(eg, xt, yt);
-    if (entry < 0) {  // only tail samples can miss
-      cnt.missed++;
-      continue;
+  for (int r0 = 0; r0 < total; r0 += 64) {
+    double v = 0.0;
+    bool ok = false;
+    if (r0 + lane < total) v = deferred_walk(d, deferred_sample(d, r0 + lane), rec64, eg, g, ok, cnt);
+    const int m = min(64, total - r0);
+    for (int i = 0; i < m; ++i) {
+      sum += __shfl(v, i, 64);
+      n_ok += __shfl((int)ok, i, 64);
     }
-    int crossed;
-    sum += walk64(rec64, entry, xt, yt, g, key, crossed, cnt);
-    cnt.fp64_redo++;
-    cnt.tet_steps += (unsigned)crossed;
-    if (crossed == 0) cnt.failed++;
-    n_ok += crossed > 0;
   }
   return sum;
 }
+#endif
 
 }  // namespace sdtfe

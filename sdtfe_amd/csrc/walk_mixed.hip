@@ -15,8 +15,10 @@ __device__ inline void load_faces(uint8_t *faces) {
 __global__ void defer_reset_kernel(int *count) { *count = 0; }
 
 template <bool kFullSlab>
-__global__ __launch_bounds__(256) void map_mixed_kernel(const Rec32 *__restrict__ rec32,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDTFE_MIXED_WAVES))) void
+map_mixed_kernel(const Rec32 *__restrict__ rec32,
                                                        const Rec64 *__restrict__ rec64,
+                                                       const FaceRec *__restrict__ frec,
                                                        EntryGrid eg, MapGeom g, MixedConsts mc,
                                                        double *out, int *navg, int accumulate,
                                                        DeferQueue dq,
@@ -41,7 +43,7 @@ __global__ __launch_bounds__(256) void map_mixed_kernel(const Rec32 *__restrict_
       }
       double v;
       int crossed;
-      if (walk_mixed<kFullSlab>(rec32, rec64, faces, entry, xt, yt, g, mc.flt, v, crossed,
+      if (walk_mixed<kFullSlab>(rec32, rec64, frec, faces, entry, xt, yt, g, mc.flt, v, crossed,
                                 cnt)) {
         ps.add(v, crossed);
         cnt.tet_steps += (unsigned)crossed;
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(256) void map_mixed_kernel(const Rec32 *__restrict_
   cnt.flush(stats);
 }
 
-// Second pass: grid-stride over the queued pixels.
+// Second pass: one wave per queued pixel, grid-stride over the queue.
 __global__ __launch_bounds__(256) void map_deferred_kernel(const Rec64 *__restrict__ rec64,
                                                           EntryGrid eg, MapGeom g, double *out,
                                                           int *navg, int accumulate,
@@ -72,16 +74,19 @@ __global__ __launch_bounds__(256) void map_deferred_kernel(const Rec64 *__restri
                                                           unsigned long long *stats) {
   LaneCounters cnt;
   const int n = min(*dq.count, dq.cap);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const int waves = gridDim.x * (blockDim.x >> 6);
+  for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
     int n_ok;
-    const double sum = finish_deferred(dq.q + i, rec64, eg, g, n_ok, cnt);
-    const uint64_t pix = dq.q[i].pix;
-    const double val = n_ok > 0 ? sum / n_ok : 0.0;
-    if (accumulate)
-      out[pix] += val;
-    else
-      out[pix] = val;
-    if (navg) navg[pix] = n_ok;
+    const double sum = finish_deferred_wave(dq.q + i, rec64, eg, g, n_ok, cnt);
+    if ((threadIdx.x & 63) == 0) {
+      const uint64_t pix = dq.q[i].pix;
+      const double val = n_ok > 0 ? sum / n_ok : 0.0;
+      if (accumulate)
+        out[pix] += val;
+      else
+        out[pix] = val;
+      if (navg) navg[pix] = n_ok;
+    }
   }
   cnt.flush(stats);
 }
@@ -89,7 +94,8 @@ __global__ __launch_bounds__(256) void map_deferred_kernel(const Rec64 *__restri
 // Rays one by one; a handed-over ray is walked in FP64 in place (this
 // kernel serves inspection, not maps).
 __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
-                                   const Rec64 *__restrict__ rec64, EntryGrid eg, MapGeom g,
+                                   const Rec64 *__restrict__ rec64,
+                                   const FaceRec *__restrict__ frec, EntryGrid eg, MapGeom g,
                                    MixedConsts mc, int n, const double *x, const double *y,
                                    const int *entry_in, const uint64_t *pixel,
                                    const uint32_t *sample, double *value, int *crossed_out,
@@ -104,9 +110,9 @@ __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
     int crossed;
     double v;
     const bool done = mc.flt.full_slab
-                          ? walk_mixed<true>(rec32, rec64, faces, entry, x[i], y[i], g, mc.flt,
+                          ? walk_mixed<true>(rec32, rec64, frec, faces, entry, x[i], y[i], g, mc.flt,
                                              v, crossed, cnt)
-                          : walk_mixed<false>(rec32, rec64, faces, entry, x[i], y[i], g, mc.flt,
+                          : walk_mixed<false>(rec32, rec64, frec, faces, entry, x[i], y[i], g, mc.flt,
                                               v, crossed, cnt);
     if (!done) {
       cnt.fp64_redo++;
@@ -155,10 +161,10 @@ int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
     SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
     if (mc.flt.full_slab)
       map_mixed_kernel<true><<<nb * nb, 256, 0, t->stream>>>(
-          t->rec32, t->rec64, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
     else
       map_mixed_kernel<false><<<nb * nb, 256, 0, t->stream>>>(
-          t->rec32, t->rec64, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
     SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
     map_deferred_kernel<<<1024, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg,
                                                      accumulate, dq, t->stats);
@@ -179,7 +185,7 @@ void launch_trace_mixed(sdtfe_tess *t, const MapGeom &g, int n, const double *x,
                         const double *y, const int *entry, const uint64_t *pixel,
                         const uint32_t *sample, double *value, int *crossed) {
   trace_mixed_kernel<<<(n + 255) / 256, 256, 0, t->stream>>>(
-      t->rec32, t->rec64, t->grid, g, mixed_consts(t, g), n, x, y, entry, pixel,
+      t->rec32, t->rec64, t->facerec, t->grid, g, mixed_consts(t, g), n, x, y, entry, pixel,
       sample, value, crossed, t->stats);
 }
 

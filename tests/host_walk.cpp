@@ -50,10 +50,13 @@ int main(int argc, char **argv) {
       fread(ent.data(), 8, (size_t)n, f) != (size_t)n)
     return 6;
   std::vector<sdtfe::Rec32> rec32;
+  std::vector<sdtfe::FaceRec> frec;
   double org_span[6] = {0, 0, 0, 0, 0, 0};  // origin, half-extent, z range
   if (mixed) {
     rec32.resize((size_t)nt);
+    frec.resize(4 * (size_t)nt);
     if (fread(rec32.data(), sizeof(sdtfe::Rec32), (size_t)nt, f) != (size_t)nt ||
+        fread(frec.data(), sizeof(sdtfe::FaceRec), 4 * (size_t)nt, f) != 4 * (size_t)nt ||
         fread(org_span, 8, 6, f) != 6)
       return 8;
   }
@@ -91,9 +94,9 @@ int main(int argc, char **argv) {
     // the mixed walk hands some rays to the FP64 walk, as the map kernels do
     const bool done =
         mixed && (mc.flt.full_slab
-                      ? sdtfe::walk_mixed<true>(rec32.data(), rec.data(), faces, (int)ent[i], x[i],
+                      ? sdtfe::walk_mixed<true>(rec32.data(), rec.data(), frec.data(), faces, (int)ent[i], x[i],
                                                 y[i], g, mc.flt, v, crossed, c)
-                      : sdtfe::walk_mixed<false>(rec32.data(), rec.data(), faces, (int)ent[i], x[i],
+                      : sdtfe::walk_mixed<false>(rec32.data(), rec.data(), frec.data(), faces, (int)ent[i], x[i],
                                                  y[i], g, mc.flt, v, crossed, c));
     if (!done) {
       if (mixed) c.fp64_redo++;

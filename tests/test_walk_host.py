@@ -51,6 +51,42 @@ def records32(pos, tetra, rho):
     return rec
 
 
+def face_records(pos, tetra, rho):
+    """The mixed walk's 32-byte face records (4 per tet): the tet across
+    face f, the face's index there plus the labels there of this tet's
+    vertices f^1, f^2, f^3, and the new vertex (x, y FP64; z relative to
+    vertex f^1 and density FP32)."""
+    t = tetra.shape[0]
+    rec = np.zeros((t, 4), dtype=np.dtype([("nb", "<i4"), ("code", "<u4"), ("qx", "<f8"),
+                                           ("qy", "<f8"), ("dqz", "<f4"), ("rho", "<f4")]))
+    assert rec.dtype.itemsize == 32
+    verts, nbrs = tetra[:, :4], tetra[:, 4:]
+    for f in range(4):
+        n = nbrs[:, f]
+        rec["nb"][:, f] = n
+        ok = n >= 0
+        nn = np.where(ok, n, 0)
+        back = np.full(t, -1)
+        for k in range(3, -1, -1):
+            back = np.where(nbrs[nn, k] == np.arange(t), k, back)
+        code = np.where(back < 0, 1 << 8, back & 3).astype(np.int64)
+        for k in (1, 2, 3):
+            v = verts[:, f ^ k]
+            lab = np.full(t, -1)
+            for m in range(3, -1, -1):
+                lab = np.where(verts[nn, m] == v, m, lab)
+            code |= np.where((lab < 0) | (lab == back), 1 << 8, 0)
+            code |= (lab & 3) << (2 * k)
+        q = verts[nn, back & 3]
+        ref = verts[:, f ^ 1]
+        rec["code"][:, f] = np.where(ok, code, 0)
+        rec["qx"][:, f] = np.where(ok, pos[q, 0], 0)
+        rec["qy"][:, f] = np.where(ok, pos[q, 1], 0)
+        rec["dqz"][:, f] = np.where(ok, (pos[q, 2] - pos[ref, 2]).astype(np.float32), 0)
+        rec["rho"][:, f] = np.where(ok, rho[q].astype(np.float32), 0)
+    return rec.reshape(-1)
+
+
 def run_walk(host_walk, tmp_path, g, mixed=False):
     rec = records(g["pos"], g["tetra"], g["rho"], g["nabla"])
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
@@ -68,6 +104,7 @@ def run_walk(host_walk, tmp_path, g, mixed=False):
             lo, hi = g["pos"].min(0), g["pos"].max(0)
             org = 0.5 * (lo + hi)
             records32(g["pos"], g["tetra"], g["rho"]).tofile(f)
+            face_records(g["pos"], g["tetra"], g["rho"]).tofile(f)
             zr = (lo[2], hi[2])
             np.array([*org, float((0.5 * (hi - lo)).max()), *zr]).tofile(f)
     r = subprocess.run([host_walk, str(inp), str(out)] + (["mixed"] if mixed else []),
