@@ -103,8 +103,22 @@ SDTFE_HD inline float side32(float x1, float y1, float x2, float y2) {
   return fmaf(x1, y2, -(y1 * x2));
 }
 
-SDTFE_HD inline float absmax4(float a, float b, float c, float d) {
-  return fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d)));
+SDTFE_HD inline float absmax3(float a, float b, float c) {
+  return fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(c));
+}
+
+// A face record as two 16-byte loads issued together.
+SDTFE_HD inline FaceRec load_facerec(const FaceRec *__restrict__ frec, int64_t i) {
+  const uint4 *p = reinterpret_cast<const uint4 *>(frec + i);
+  const uint4 a = p[0], b = p[1];
+  FaceRec r;
+  r.nb = (int)a.x;
+  r.code = a.y;
+  r.qx = __builtin_bit_cast(double, ((uint64_t)a.w << 32) | a.z);
+  r.qy = __builtin_bit_cast(double, ((uint64_t)b.y << 32) | b.x);
+  r.dqz = __builtin_bit_cast(float, b.z);
+  r.rho = __builtin_bit_cast(float, b.w);
+  return r;
 }
 
 // Vertex densities of tet t (FP32, from the mixed record).
@@ -252,7 +266,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     const float bx = (float)(q.bx - x), by = (float)(q.by - y);
     const float cx = (float)(q.cx - x), cy = (float)(q.cy - y);
     const float dx = (float)(q.dx - x), dy = (float)(q.dy - y);
-    const float M = fmaxf(absmax4(ax, ay, bx, by), absmax4(cx, cy, dx, dy));
+    const float M = fmaxf(fmaxf(absmax3(ax, ay, bx), absmax3(by, cx, cy)), absmax3(dx, dy, 0.f));
     const float bound = fmaf(flt.k_err, M * M, flt.thr);
     const float cs[6] = {side32(cx, cy, bx, by), side32(cx, cy, ax, ay), side32(bx, by, ax, ay),
                          side32(cx, cy, dx, dy), side32(bx, by, dx, dy), side32(dx, dy, ax, ay)};
@@ -300,24 +314,26 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     crossed = 1;
   }
 
-  // ---- the walk: leave t through face f (the carried face)
+  // ---- the walk: leave t through face f (the carried face).  The next
+  // face record is requested as soon as f is known, so its load overlaps
+  // the face update and the exit crossing.
   double acc = 0.0;
   int steps = 1;
   bool last = false;
+  FaceRec R = load_facerec(frec, 4 * (int64_t)t + f);
   for (;;) {
-    const FaceRec R = frec[4 * (int64_t)t + f];
+    // exit crossing in FP32, relative to P0 (does not wait for R)
+    const float den = (e0 + e1) + e2;
+    const float M = fmaxf(absmax3(p0.ox, p0.oy, p1.ox), absmax3(p1.oy, p2.ox, p2.oy));
+    const float inv = rcp32(den);
+    const float dz1 = (float)(p1.z - p0.z), dz2 = (float)(p2.z - p0.z);
+    double z_out = p0.z + (double)(fmaf(e2, dz1, e0 * dz2) * inv);
+    float r_out = fmaf(e1, p0.rho, fmaf(e2, p1.rho, e0 * p2.rho)) * inv;
     if (R.nb < 0) {  // leaves the hull: last crossing below
       last = true;
       break;
     }
     if (R.code & kFaceRecBad) break;
-    // exit crossing in FP32, relative to P0
-    const float den = (e0 + e1) + e2;
-    const float M = fmaxf(absmax4(p0.ox, p0.oy, p1.ox, p1.oy), fmaxf(fabsf(p2.ox), fabsf(p2.oy)));
-    const float inv = rcp32(den);
-    const float dz1 = (float)(p1.z - p0.z), dz2 = (float)(p2.z - p0.z);
-    double z_out = p0.z + (double)(fmaf(e2, dz1, e0 * dz2) * inv);
-    float r_out = fmaf(e1, p0.rho, fmaf(e2, p1.rho, e0 * p2.rho)) * inv;
     if (fabsf(den) < flt.k_ill * M * M) {
       // rare: FP32 weights of a nearly vertical face are too uncertain
       const Cross64 o = face_cross64(rec64, t, f, x, y, load_rho4(rec32, t));
@@ -356,7 +372,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     float s0 = side32(p0.ox, p0.oy, qv.ox, qv.oy);
     float s1 = side32(p1.ox, p1.oy, qv.ox, qv.oy);
     float s2 = side32(p2.ox, p2.oy, qv.ox, qv.oy);
-    const float Mt = fmaxf(M, fmaxf(fabsf(qv.ox), fabsf(qv.oy)));
+    const float Mt = fmaxf(fmaxf(M, fabsf(qv.ox)), fabsf(qv.oy));
     const float bound = fmaf(flt.k_err, Mt * Mt, flt.thr);
     if (!(fminf(fminf(fabsf(s0), fabsf(s1)), fabsf(s2)) > bound)) {
       // rare: the FP64 walk's signs for the sides the filter leaves open
@@ -383,6 +399,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     const bool r2 = n0 && !n1, r0 = n1 && !n2, r1 = n2 && !n0;
     if ((int)r0 + (int)r1 + (int)r2 != 1) break;
     f = seli(r0, p0.lab, seli(r1, p1.lab, p2.lab));
+    R = load_facerec(frec, 4 * (int64_t)t + f);
     if (!kFullSlab)
       zext = (float)(fmax(fmax(p0.z, p1.z), fmax(p2.z, qv.z)) -
                      fmin(fmin(p0.z, p1.z), fmin(p2.z, qv.z)));
