@@ -42,20 +42,20 @@ struct alignas(16) Rec32 {
 };
 static_assert(sizeof(Rec32) == 96, "Rec32 layout");
 
-// Face record of the mixed walk, one per (tet, face), 32 B: where a ray
-// that leaves tet t through face f (the face opposite vertex f) goes.  The
-// walk carries the face's three vertices, so the next tet only adds its
-// fourth vertex, stored here.
+// Face record of the mixed walk, one per (tet, face), 32 B read as two
+// 16-byte loads: where a ray that leaves tet t through face f (the face
+// opposite vertex f) goes.  The walk carries the face's three vertices, so
+// the next tet only adds its fourth vertex, stored here.
+//   q     the new vertex: x, y in FP64; z minus that of t's vertex f^1 and
+//         the density in FP32
 //   nb    tet across the face (-1 on the hull)
 //   code  bits 0-1: the face's index in nb (nb's vertex opposite it, i.e.
 //         the new vertex); bits 2-7: the labels in nb of t's vertices f^1,
 //         f^2, f^3 (2 bits each); bit 8: the mesh lacks the back link
-//   q     the new vertex: x, y in FP64, z minus that of t's vertex f^1 in
-//         FP32, and its density
 struct alignas(16) FaceRec {
+  double qx, qy;
   int nb;
   unsigned code;
-  double qx, qy;
   float dqz;
   float rho;
 };

@@ -112,10 +112,10 @@ SDTFE_HD inline FaceRec load_facerec(const FaceRec *__restrict__ frec, int64_t i
   const uint4 *p = reinterpret_cast<const uint4 *>(frec + i);
   const uint4 a = p[0], b = p[1];
   FaceRec r;
-  r.nb = (int)a.x;
-  r.code = a.y;
-  r.qx = __builtin_bit_cast(double, ((uint64_t)a.w << 32) | a.z);
-  r.qy = __builtin_bit_cast(double, ((uint64_t)b.y << 32) | b.x);
+  r.qx = __builtin_bit_cast(double, ((uint64_t)a.y << 32) | a.x);
+  r.qy = __builtin_bit_cast(double, ((uint64_t)a.w << 32) | a.z);
+  r.nb = (int)b.x;
+  r.code = b.y;
   r.dqz = __builtin_bit_cast(float, b.z);
   r.rho = __builtin_bit_cast(float, b.w);
   return r;
@@ -397,7 +397,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     // the face replacing P_j: j = 2 if s0 < 0 < s1, 0 if s1 < 0 < s2, 1 if s2 < 0 < s0
     const bool n0 = __builtin_signbit(s0), n1 = __builtin_signbit(s1), n2 = __builtin_signbit(s2);
     const bool r2 = n0 && !n1, r0 = n1 && !n2, r1 = n2 && !n0;
-    if ((int)r0 + (int)r1 + (int)r2 != 1) break;
+    if (!(r0 || r1 || r2)) break;  // (with exact signs exactly one holds)
     f = seli(r0, p0.lab, seli(r1, p1.lab, p2.lab));
     R = load_facerec(frec, 4 * (int64_t)t + f);
     if (!kFullSlab)

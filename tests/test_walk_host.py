@@ -52,13 +52,13 @@ def records32(pos, tetra, rho):
 
 
 def face_records(pos, tetra, rho):
-    """The mixed walk's 32-byte face records (4 per tet): the tet across
-    face f, the face's index there plus the labels there of this tet's
-    vertices f^1, f^2, f^3, and the new vertex (x, y FP64; z relative to
-    vertex f^1 and density FP32)."""
+    """The mixed walk's 32-byte face records (4 per tet): the new vertex
+    (x, y FP64), the tet across face f, the face's index there plus the
+    labels there of this tet's vertices f^1, f^2, f^3, and the new vertex's
+    z relative to vertex f^1 and density (FP32)."""
     t = tetra.shape[0]
-    rec = np.zeros((t, 4), dtype=np.dtype([("nb", "<i4"), ("code", "<u4"), ("qx", "<f8"),
-                                           ("qy", "<f8"), ("dqz", "<f4"), ("rho", "<f4")]))
+    rec = np.zeros((t, 4), dtype=np.dtype([("qx", "<f8"), ("qy", "<f8"), ("nb", "<i4"),
+                                           ("code", "<u4"), ("dqz", "<f4"), ("rho", "<f4")]))
     assert rec.dtype.itemsize == 32
     verts, nbrs = tetra[:, :4], tetra[:, 4:]
     for f in range(4):
