@@ -199,11 +199,11 @@ SDTFE_HD inline bool slab_inside(const Rec64 *__restrict__ rec64, const Rec32 *_
   return true;
 }
 
-// One vertex of the carried face: offset from the ray, height, density and
-// its label (0..3 for A..D) in the current tet.
+// One vertex of the carried face: offset from the ray, height above the
+// carried crossing (both FP32, so their rounding is a fraction of the local
+// tet size), density and its label (0..3 for A..D) in the current tet.
 struct Slot {
-  float ox, oy;
-  double z;
+  float ox, oy, z;
   float rho;
   int lab;
 };
@@ -214,20 +214,12 @@ SDTFE_HD inline float self(bool m, float a, float b) {
                                           __builtin_bit_cast(uint32_t, b)));
 }
 
-SDTFE_HD inline double seld(bool m, double a, double b) {
-  const uint32_t k = 0u - (uint32_t)m;
-  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
-  const uint32_t lo = bsel32(k, (uint32_t)ua, (uint32_t)ub);
-  const uint32_t hi = bsel32(k, (uint32_t)(ua >> 32), (uint32_t)(ub >> 32));
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
 SDTFE_HD inline int seli(bool m, int a, int b) {
   return (int)bsel32(0u - (uint32_t)m, (uint32_t)a, (uint32_t)b);
 }
 
 SDTFE_HD inline Slot sel_slot(bool m, const Slot &a, const Slot &b) {
-  return Slot{self(m, a.ox, b.ox), self(m, a.oy, b.oy), seld(m, a.z, b.z), self(m, a.rho, b.rho),
+  return Slot{self(m, a.ox, b.ox), self(m, a.oy, b.oy), self(m, a.z, b.z), self(m, a.rho, b.rho),
               seli(m, a.lab, b.lab)};
 }
 
@@ -297,21 +289,21 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     // magnitudes keep that sign also where a side's FP32 value was too small
     // to be decided
     const float oxs[4] = {ax, bx, cx, dx}, oys[4] = {ay, by, cy, dy};
-    const double zs[4] = {q.az, q.bz, q.cz, q.dz};
+    const float zs[4] = {(float)(q.az - zc), (float)(q.bz - zc), (float)(q.cz - zc),
+                         (float)(q.dz - zc)};
     const int l0 = f ^ 1, l1 = f ^ 3, l2 = f ^ 2;
     p0 = Slot{pick4f(l0, oxs[0], oxs[1], oxs[2], oxs[3]), pick4f(l0, oys[0], oys[1], oys[2], oys[3]),
-              pick4(l0, zs[0], zs[1], zs[2], zs[3]),
+              pick4f(l0, zs[0], zs[1], zs[2], zs[3]),
               pick4f(l0, rho.v[0], rho.v[1], rho.v[2], rho.v[3]), l0};
     p1 = Slot{pick4f(l1, oxs[0], oxs[1], oxs[2], oxs[3]), pick4f(l1, oys[0], oys[1], oys[2], oys[3]),
-              pick4(l1, zs[0], zs[1], zs[2], zs[3]),
+              pick4f(l1, zs[0], zs[1], zs[2], zs[3]),
               pick4f(l1, rho.v[0], rho.v[1], rho.v[2], rho.v[3]), l1};
     p2 = Slot{pick4f(l2, oxs[0], oxs[1], oxs[2], oxs[3]), pick4f(l2, oys[0], oys[1], oys[2], oys[3]),
-              pick4(l2, zs[0], zs[1], zs[2], zs[3]),
+              pick4f(l2, zs[0], zs[1], zs[2], zs[3]),
               pick4f(l2, rho.v[0], rho.v[1], rho.v[2], rho.v[3]), l2};
     e0 = fabsf(side32(p0.ox, p0.oy, p1.ox, p1.oy));
     e1 = fabsf(side32(p1.ox, p1.oy, p2.ox, p2.oy));
     e2 = fabsf(side32(p2.ox, p2.oy, p0.ox, p0.oy));
-    crossed = 1;
   }
 
   // ---- the walk: leave t through face f (the carried face).  The next
@@ -322,12 +314,12 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
   bool last = false;
   FaceRec R = load_facerec(frec, 4 * (int64_t)t + f);
   for (;;) {
-    // exit crossing in FP32, relative to P0 (does not wait for R)
+    // exit crossing in FP32, height above the entry crossing (does not wait
+    // for R)
     const float den = (e0 + e1) + e2;
     const float M = fmaxf(absmax3(p0.ox, p0.oy, p1.ox), absmax3(p1.oy, p2.ox, p2.oy));
     const float inv = rcp32(den);
-    const float dz1 = (float)(p1.z - p0.z), dz2 = (float)(p2.z - p0.z);
-    double z_out = p0.z + (double)(fmaf(e2, dz1, e0 * dz2) * inv);
+    float dz = fmaf(e1, p0.z, fmaf(e2, p1.z, e0 * p2.z)) * inv;
     float r_out = fmaf(e1, p0.rho, fmaf(e2, p1.rho, e0 * p2.rho)) * inv;
     if (R.nb < 0) {  // leaves the hull: last crossing below
       last = true;
@@ -337,19 +329,23 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     if (fabsf(den) < flt.k_ill * M * M) {
       // rare: FP32 weights of a nearly vertical face are too uncertain
       const Cross64 o = face_cross64(rec64, t, f, x, y, load_rho4(rec32, t));
-      z_out = o.z;
+      dz = (float)(o.z - zc);
       r_out = o.rho;
     }
-    const double dz = z_out - zc;
     if (kFullSlab) {
-      acc += (double)(0.5f * (rc + r_out)) * dz;
+      acc += (double)(0.5f * (rc + r_out) * dz);
     } else {
       bool inside;
-      if (!slab_inside(rec64, rec32, t, enter, f, x, y, zext, zc, z_out, g, flt, inside)) break;
-      if (inside) acc += (double)(0.5f * (rc + r_out)) * dz;
+      if (!slab_inside(rec64, rec32, t, enter, f, x, y, zext, zc, zc + (double)dz, g, flt,
+                       inside))
+        break;
+      if (inside) acc += (double)(0.5f * (rc + r_out) * dz);
     }
-    zc = z_out;
+    zc += (double)dz;
     rc = r_out;
+    p0.z -= dz;
+    p1.z -= dz;
+    p2.z -= dz;
 
     // enter the next tet through its face `back`; its fourth vertex Q
     if (++steps > g.max_steps) {
@@ -357,10 +353,9 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
       crossed = 0;
       return true;
     }
-    ++crossed;
     const int back = (int)(R.code & 3u);
-    const double zref = seld(p0.lab == (f ^ 1), p0.z, seld(p1.lab == (f ^ 1), p1.z, p2.z));
-    Slot qv{(float)(R.qx - x), (float)(R.qy - y), zref + (double)R.dqz, R.rho, back};
+    const float zref = self(p0.lab == (f ^ 1), p0.z, self(p1.lab == (f ^ 1), p1.z, p2.z));
+    Slot qv{(float)(R.qx - x), (float)(R.qy - y), zref + R.dqz, R.rho, back};
     // labels of the carried vertices in the next tet
     p0.lab = (int)((R.code >> (2 * (p0.lab ^ f))) & 3u);
     p1.lab = (int)((R.code >> (2 * (p1.lab ^ f))) & 3u);
@@ -401,8 +396,8 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     f = seli(r0, p0.lab, seli(r1, p1.lab, p2.lab));
     R = load_facerec(frec, 4 * (int64_t)t + f);
     if (!kFullSlab)
-      zext = (float)(fmax(fmax(p0.z, p1.z), fmax(p2.z, qv.z)) -
-                     fmin(fmin(p0.z, p1.z), fmin(p2.z, qv.z)));
+      zext = fmaxf(fmaxf(p0.z, p1.z), fmaxf(p2.z, qv.z)) -
+             fminf(fminf(p0.z, p1.z), fminf(p2.z, qv.z));
     // new face: P_j := Q; sides (P_{j-1}, Q) = s_{j-1}, (Q, P_{j+1}) = -s_{j+1}
     const float ne0 = self(r1, s0, self(r0, -s1, e0));
     const float ne1 = self(r2, s1, self(r1, -s2, e1));
@@ -426,6 +421,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
   }
   if (inside) acc += (double)(0.5f * (rc + out.rho)) * dz;
   value = acc;
+  crossed = steps;
   return true;
 }
 
