@@ -256,6 +256,9 @@ def main():
         # cache reuse between neighbouring rays lifts the algorithmic rate
         # past the HBM peak: the kernel is not HBM-bound
         roofline["hbm_bound"] = False
+        roofline["note"] = ("algorithmic bytes/s exceed the HBM peak because the rays of a wave "
+                            "re-read the same tets from L2/MALL; physical HBM traffic is "
+                            "physical_gbs; the walk is bound by instruction issue")
 
     cpu = None
     if not a.no_cpu:
