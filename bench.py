@@ -172,12 +172,109 @@ def copy_calibration(gib: float = 1.0, reps: int = 3):
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "reps": reps}))
 
 
+def _precision(a):
+    from sdtfe_amd import capi
+    return capi.PREC_FP64 if a.precision == "fp64" else capi.PREC_MIXED
+
+
+def bench_cutouts(a):
+    """Batched cutouts: many small maps, each its own resident tessellation,
+    in one persistent launch (sdtfe_batch) against the same maps one launch
+    at a time.  Cutout i: particles log-uniform in [1e3, 2e5], grid
+    log-uniform in [64, 1024] (multiple of 4), 4-16 samples per pixel."""
+    from sdtfe_amd import capi, serve
+    prec = _precision(a)
+    t0 = time.perf_counter()
+    reqs = serve.make_requests(a.cutouts, seed=a.seed, workers=16)
+    log(f"[bench] {len(reqs)} cutouts tessellated in {time.perf_counter() - t0:.1f} s, "
+        f"{sum(r.pd.shape[0] for r in reqs)} particles, {sum(r.tetra.shape[0] for r in reqs)} tets")
+    tess = [capi.Tessellation(r.pd, r.tetra) for r in reqs]
+    bufs = [capi.DeviceBuffer(r.grid * r.grid * 8) for r in reqs]
+    params = [t.params(r.grid, r.samples, seed=r.seed, precision=prec) for t, r in zip(tess, reqs)]
+    lpp = [4 if r.grid <= 256 else 1 for r in reqs]
+    rays = sum(r.rays for r in reqs)
+    for t in tess:
+        t.sync()
+    with capi.Batch() as b:
+        items = list(zip(tess, params, bufs))
+        for _ in range(a.warmup):
+            b.run(items, lanes_per_pixel=lpp)
+        walk, wall = [], []
+        for _ in range(a.steps):
+            for buf in bufs:
+                buf.zero()
+            w0 = time.perf_counter()
+            _, tot = b.run(items, lanes_per_pixel=lpp)
+            wall.append(time.perf_counter() - w0)
+            walk.append(tot["t_walk_ms"])
+    # the same maps one launch each
+    w0 = time.perf_counter()
+    for t, p, buf in zip(tess, params, bufs):
+        t.map_device(p, buf, stats=False)
+    for t in tess:
+        t.sync()
+    single_s = time.perf_counter() - w0
+    for t in tess:
+        t.close()
+    for buf in bufs:
+        buf.close()
+    ms = 1e3 * float(np.mean(wall))
+    out = {"metric": "sample-rays/s", "value": round(rays / (ms / 1e3), 1),
+           "unit": "sample-rays/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64" if prec == capi.PREC_MIXED else "f64",
+           "data": "synthetic",
+           "config": {"workload": f"{len(reqs)} halo cutouts (1e3-2e5 particles, 64-1024 px, "
+                                  f"4-16 samples/pixel) in one batched launch",
+                      "sample_rays": rays, "maps": len(reqs)},
+           "batched_walk_ms": round(float(np.mean(walk)), 3),
+           "one_launch_per_map_s": round(single_s, 3),
+           "batched_over_one_by_one": round(single_s / (ms / 1e3), 2)}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(a):
+    """Serving: a Poisson stream of cutout requests (particles, tets, map
+    parameters) against the continuous-batching server; first all at once
+    (capacity), then at 75% of that rate.  Reports p50/p99 latency from
+    submission to the map on the host."""
+    from sdtfe_amd import capi, serve
+    prec = _precision(a)
+    t0 = time.perf_counter()
+    reqs = serve.make_requests(a.cutouts, seed=a.seed, workers=16)
+    log(f"[bench] {len(reqs)} requests tessellated in {time.perf_counter() - t0:.1f} s")
+    with capi.Server(hbm_budget_bytes=64 << 30, max_active=64, tile_budget=8192,
+                     precision=prec) as srv:
+        cap = serve.run_stream(srv, reqs, None, seed=a.seed)
+    rate = 0.75 * cap["requests"] / cap["wall_s"]
+    log(f"[bench] capacity {cap['requests'] / cap['wall_s']:.1f} maps/s; stream at {rate:.1f}/s")
+    with capi.Server(hbm_budget_bytes=64 << 30, max_active=64, tile_budget=8192,
+                     precision=prec) as srv:
+        res = serve.run_stream(srv, reqs, rate, seed=a.seed + 1)
+    out = {"metric": "sample-rays/s", "value": round(res["sample_rays_per_s"], 1),
+           "unit": "sample-rays/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "ms_per_step": round(1e3 * res["wall_s"], 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64" if prec == capi.PREC_MIXED else "f64", "data": "synthetic",
+           "config": {"workload": f"Poisson stream of {len(reqs)} cutout requests at "
+                                  f"{rate:.1f}/s (75% of measured capacity)",
+                      "requests": len(reqs)},
+           "p50_ms": round(res["p50_ms"], 2), "p99_ms": round(res["p99_ms"], 2),
+           "mean_ms": round(res["mean_ms"], 2), "queue_p99_ms": round(res["queue_p99_ms"], 2),
+           "capacity_maps_per_s": round(cap["requests"] / cap["wall_s"], 2),
+           "capacity_sample_rays_per_s": round(cap["sample_rays_per_s"], 1),
+           "server": res["server"]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="clustered", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="clustered",
+                    choices=sorted(CONFIGS) + ["cutouts", "serve"])
+    ap.add_argument("--cutouts", type=int, default=512, help="cutouts / requests (cutouts, serve)")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -194,6 +291,10 @@ def main():
               "whole maps or 4-row bands, which this build does not run across GPUs.",
               file=sys.stderr)
         sys.exit(2)
+
+    if a.config in ("cutouts", "serve"):
+        (bench_cutouts if a.config == "cutouts" else bench_serve)(a)
+        return
 
     from sdtfe_amd import capi
     from sdtfe_amd.tessellate import to_particle_data
