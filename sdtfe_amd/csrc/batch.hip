@@ -98,10 +98,10 @@ __device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int
   }
   if (kMixed) {
     const bool done = it.mc.flt.full_slab
-                          ? walk_mixed<true>(it.rec32, it.rec64, it.facerec, faces, entry, xt, yt, g,
-                                             it.mc.flt, v, crossed, cnt)
-                          : walk_mixed<false>(it.rec32, it.rec64, it.facerec, faces, entry, xt, yt, g,
-                                              it.mc.flt, v, crossed, cnt);
+                          ? walk_mixed<true, false>(it.rec32, it.rec64, it.facerec, faces, entry,
+                                                    xt, yt, g, it.mc.flt, v, crossed, cnt)
+                          : walk_mixed<false, false>(it.rec32, it.rec64, it.facerec, faces, entry,
+                                                     xt, yt, g, it.mc.flt, v, crossed, cnt);
     if (!done) return false;
   } else {
     v = walk64(it.rec64, entry, xt, yt, g, key, crossed, cnt);
@@ -380,6 +380,11 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
   }
   if (prec != SDTFE_PREC_FP64 && prec != SDTFE_PREC_MIXED)
     return fail(SDTFE_EINVAL, "unknown precision");
+  // the batched mixed walk addresses face records with 32-bit offsets
+  if (prec == SDTFE_PREC_MIXED)
+    for (int i = 0; i < n; ++i)
+      if ((int64_t)tess[i]->nt > kFaceRecNarrowTets)
+        return fail(SDTFE_EINVAL, "tessellation too large for a batched mixed map; map it alone");
   int rc = ensure_cap(b, n);
   if (rc) return rc;
   const auto t0 = std::chrono::steady_clock::now();

@@ -254,7 +254,7 @@ __global__ void face_record_kernel(const double4 *pos, const int *tets, int nt, 
   const int t = (int)(i >> 2), f = (int)(i & 3);
   FaceRec r;
   r.nb = tets[8 * (int64_t)t + 4 + f];
-  r.code = 0;
+  r.code = kFaceStop;
   r.qx = r.qy = 0.0;
   r.dqz = r.rho = 0.f;
   if (r.nb >= 0) {
@@ -262,21 +262,22 @@ __global__ void face_record_kernel(const double4 *pos, const int *tets, int nt, 
     int back = -1;
     for (int k = 3; k >= 0; --k)
       if (tets[8 * (int64_t)n + 4 + k] == t) back = k;
-    unsigned code = back < 0 ? kFaceRecBad : (unsigned)back;
+    unsigned code = back < 0 ? kFaceStop : (unsigned)back << 3;
+    double zs = 0.0;
     for (int k = 1; k <= 3; ++k) {
       const int v = tets[8 * (int64_t)t + (f ^ k)];
+      zs += pos[v].z;
       int lab = -1;
       for (int m = 3; m >= 0; --m)
         if (tets[8 * (int64_t)n + m] == v) lab = m;
-      if (lab < 0 || lab == back) code |= kFaceRecBad;
-      code |= (unsigned)(lab & 3) << (2 * k);
+      if (lab < 0 || lab == back) code |= kFaceStop;
+      code |= (unsigned)(lab & 3) << (8 * k + 3);
     }
     r.code = code;
     const double4 q = pos[tets[8 * (int64_t)n + (back & 3)]];
-    const double4 ref = pos[tets[8 * (int64_t)t + (f ^ 1)]];
     r.qx = q.x;
     r.qy = q.y;
-    r.dqz = (float)(q.z - ref.z);
+    r.dqz = (float)(q.z - zs / 3.0);
     r.rho = (float)q.w;
   }
   out[i] = r;

@@ -46,12 +46,15 @@ static_assert(sizeof(Rec32) == 96, "Rec32 layout");
 // 16-byte loads: where a ray that leaves tet t through face f (the face
 // opposite vertex f) goes.  The walk carries the face's three vertices, so
 // the next tet only adds its fourth vertex, stored here.
-//   q     the new vertex: x, y in FP64; z minus that of t's vertex f^1 and
-//         the density in FP32
+//   q     the new vertex: x, y in FP64; z minus the mean z of face f's three
+//         vertices ((z1 + z2) + z3) / 3 and the density, in FP32
 //   nb    tet across the face (-1 on the hull)
-//   code  bits 0-1: the face's index in nb (nb's vertex opposite it, i.e.
-//         the new vertex); bits 2-7: the labels in nb of t's vertices f^1,
-//         f^2, f^3 (2 bits each); bit 8: the mesh lacks the back link
+//   code  8 x the face's index in nb (nb's vertex opposite it, i.e. the new
+//         vertex) in bits 3-4; 8 x the label in nb of t's vertex f^k in
+//         bits 8k..8k+4 (k = 1, 2, 3), so that the walk, which keeps
+//         labels as 8 x label, relabels a vertex with one bit-field extract
+//         at offset 8 (label ^ f); bit 31 (kFaceStop): the ray stops here
+//         (nb < 0, or the mesh lacks the back link)
 struct alignas(16) FaceRec {
   double qx, qy;
   int nb;
@@ -60,7 +63,9 @@ struct alignas(16) FaceRec {
   float rho;
 };
 static_assert(sizeof(FaceRec) == 32, "FaceRec layout");
-constexpr unsigned kFaceRecBad = 1u << 8;
+constexpr unsigned kFaceStop = 1u << 31;
+// Face records of up to this many tets are addressed with 32-bit offsets.
+constexpr int64_t kFaceRecNarrowTets = (int64_t)1 << 25;
 
 constexpr int kDeferWindow = 256;  // handed-over samples one entry can list
 

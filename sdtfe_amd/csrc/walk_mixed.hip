@@ -14,7 +14,7 @@ __device__ inline void load_faces(uint8_t *faces) {
 
 __global__ void defer_reset_kernel(int *count) { *count = 0; }
 
-template <bool kFullSlab>
+template <bool kFullSlab, bool kWide>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDTFE_MIXED_WAVES))) void
 map_mixed_kernel(const Rec32 *__restrict__ rec32,
                                                        const Rec64 *__restrict__ rec64,
@@ -43,8 +43,8 @@ map_mixed_kernel(const Rec32 *__restrict__ rec32,
       }
       double v;
       int crossed;
-      if (walk_mixed<kFullSlab>(rec32, rec64, frec, faces, entry, xt, yt, g, mc.flt, v, crossed,
-                                cnt)) {
+      if (walk_mixed<kFullSlab, kWide>(rec32, rec64, frec, faces, entry, xt, yt, g, mc.flt, v,
+                                       crossed, cnt)) {
         ps.add(v, crossed);
         cnt.tet_steps += (unsigned)crossed;
         if (crossed == 0) cnt.failed++;
@@ -110,10 +110,10 @@ __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
     int crossed;
     double v;
     const bool done = mc.flt.full_slab
-                          ? walk_mixed<true>(rec32, rec64, frec, faces, entry, x[i], y[i], g, mc.flt,
-                                             v, crossed, cnt)
-                          : walk_mixed<false>(rec32, rec64, frec, faces, entry, x[i], y[i], g, mc.flt,
-                                              v, crossed, cnt);
+                          ? walk_mixed<true, true>(rec32, rec64, frec, faces, entry, x[i], y[i], g,
+                                                   mc.flt, v, crossed, cnt)
+                          : walk_mixed<false, true>(rec32, rec64, frec, faces, entry, x[i], y[i], g,
+                                                    mc.flt, v, crossed, cnt);
     if (!done) {
       cnt.fp64_redo++;
       v = walk64(rec64, entry, x[i], y[i], g, key, crossed, cnt);
@@ -159,11 +159,18 @@ int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
     const DeferQueue dq{t->defer_buf, t->defer_count, (int)std::min(t->defer_cap, npix)};
     defer_reset_kernel<<<1, 1, 0, t->stream>>>(t->defer_count);
     SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
-    if (mc.flt.full_slab)
-      map_mixed_kernel<true><<<nb * nb, 256, 0, t->stream>>>(
+    const bool wide = (int64_t)t->nt > kFaceRecNarrowTets;
+    if (mc.flt.full_slab && !wide)
+      map_mixed_kernel<true, false><<<nb * nb, 256, 0, t->stream>>>(
+          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+    else if (!wide)
+      map_mixed_kernel<false, false><<<nb * nb, 256, 0, t->stream>>>(
+          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+    else if (mc.flt.full_slab)
+      map_mixed_kernel<true, true><<<nb * nb, 256, 0, t->stream>>>(
           t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
     else
-      map_mixed_kernel<false><<<nb * nb, 256, 0, t->stream>>>(
+      map_mixed_kernel<false, true><<<nb * nb, 256, 0, t->stream>>>(
           t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
     SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
     map_deferred_kernel<<<1024, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg,

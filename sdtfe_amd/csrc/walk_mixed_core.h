@@ -103,13 +103,24 @@ SDTFE_HD inline float side32(float x1, float y1, float x2, float y2) {
   return fmaf(x1, y2, -(y1 * x2));
 }
 
+// Largest of two values; NaN-propagating, so the compiler adds no operand
+// canonicalisation and forms three-input maxima (v_maximum3_f32).
+SDTFE_HD inline float amax(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+
 SDTFE_HD inline float absmax3(float a, float b, float c) {
-  return fmaxf(fmaxf(fabsf(a), fabsf(b)), fabsf(c));
+  return amax(amax(fabsf(a), fabsf(b)), fabsf(c));
 }
 
-// A face record as two 16-byte loads issued together.
-SDTFE_HD inline FaceRec load_facerec(const FaceRec *__restrict__ frec, int64_t i) {
-  const uint4 *p = reinterpret_cast<const uint4 *>(frec + i);
+// Face record (t, f), f8 = 8 f, as two 16-byte loads issued together.  Below
+// kFaceRecNarrowTets tets its byte offset fits 32 bits, and the load adds it
+// to the base address itself.
+template <bool kWide>
+SDTFE_HD inline FaceRec load_facerec(const FaceRec *__restrict__ frec, int t, int f8) {
+  const char *base = reinterpret_cast<const char *>(frec);
+  const uint4 *p =
+      kWide ? reinterpret_cast<const uint4 *>(base + (((uint64_t)(uint32_t)t << 7) |
+                                                      (uint32_t)(f8 << 2)))
+            : reinterpret_cast<const uint4 *>(base + (((uint32_t)t << 7) | (uint32_t)(f8 << 2)));
   const uint4 a = p[0], b = p[1];
   FaceRec r;
   r.qx = __builtin_bit_cast(double, ((uint64_t)a.y << 32) | a.x);
@@ -201,11 +212,11 @@ SDTFE_HD inline bool slab_inside(const Rec64 *__restrict__ rec64, const Rec32 *_
 
 // One vertex of the carried face: offset from the ray, height above the
 // carried crossing (both FP32, so their rounding is a fraction of the local
-// tet size), density and its label (0..3 for A..D) in the current tet.
+// tet size), density and 8 x its label (0..3 for A..D) in the current tet.
 struct Slot {
   float ox, oy, z;
   float rho;
-  int lab;
+  int lab8;
 };
 
 // m ? a : b per lane, without control flow (see pick4).
@@ -220,7 +231,7 @@ SDTFE_HD inline int seli(bool m, int a, int b) {
 
 SDTFE_HD inline Slot sel_slot(bool m, const Slot &a, const Slot &b) {
   return Slot{self(m, a.ox, b.ox), self(m, a.oy, b.oy), self(m, a.z, b.z), self(m, a.rho, b.rho),
-              seli(m, a.lab, b.lab)};
+              seli(m, a.lab8, b.lab8)};
 }
 
 // One sample-ray.  (x, y) FP64 sample position.  Returns false when the ray
@@ -229,8 +240,9 @@ SDTFE_HD inline Slot sel_slot(bool m, const Slot &a, const Slot &b) {
 // or a midpoint on the slab boundary.  Otherwise value and crossed are the
 // ray's integral and tet count, and the tets are the FP64 walk's.  faces:
 // the kFaceCodes face_code values (LDS on the device).  kFullSlab: the slab
-// holds the hull's z range (Filter::full_slab), no midpoint test.
-template <bool kFullSlab>
+// holds the hull's z range (Filter::full_slab), no midpoint test.  kWide:
+// more than kFaceRecNarrowTets tets.
+template <bool kFullSlab, bool kWide>
 SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec64,
                                 const FaceRec *__restrict__ frec, const uint8_t *faces, int entry,
                                 double x, double y, const MapGeom &g, const Filter &flt,
@@ -247,7 +259,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
   int t = entry;
   Slot p0, p1, p2;
   float e0, e1, e2;  // sides of the carried face: (P0, P1), (P1, P2), (P2, P0)
-  int enter, f;      // t's entry face, exit face
+  int enter, f8;     // t's entry face, 8 x its exit face
   double zc;         // carried crossing: absolute height
   float rc;          // and density
   float zext;        // t's height range
@@ -258,7 +270,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     const float bx = (float)(q.bx - x), by = (float)(q.by - y);
     const float cx = (float)(q.cx - x), cy = (float)(q.cy - y);
     const float dx = (float)(q.dx - x), dy = (float)(q.dy - y);
-    const float M = fmaxf(fmaxf(absmax3(ax, ay, bx), absmax3(by, cx, cy)), absmax3(dx, dy, 0.f));
+    const float M = amax(amax(absmax3(ax, ay, bx), absmax3(by, cx, cy)), absmax3(dx, dy, 0.f));
     const float bound = fmaf(flt.k_err, M * M, flt.thr);
     const float cs[6] = {side32(cx, cy, bx, by), side32(cx, cy, ax, ay), side32(bx, by, ax, ay),
                          side32(cx, cy, dx, dy), side32(bx, by, dx, dy), side32(dx, dy, ax, ay)};
@@ -275,7 +287,8 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     const int code = faces[w];
     if (zero || !(code & 16)) return false;
     enter = code & 3;
-    f = (code >> 2) & 3;
+    const int f = (code >> 2) & 3;
+    f8 = 8 * f;
     zext = (float)(fmax(fmax(q.az, q.bz), fmax(q.cz, q.dz)) -
                    fmin(fmin(q.az, q.bz), fmin(q.cz, q.dz)));
     const Rho4 rho = load_rho4(rec32, t);
@@ -294,13 +307,13 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     const int l0 = f ^ 1, l1 = f ^ 3, l2 = f ^ 2;
     p0 = Slot{pick4f(l0, oxs[0], oxs[1], oxs[2], oxs[3]), pick4f(l0, oys[0], oys[1], oys[2], oys[3]),
               pick4f(l0, zs[0], zs[1], zs[2], zs[3]),
-              pick4f(l0, rho.v[0], rho.v[1], rho.v[2], rho.v[3]), l0};
+              pick4f(l0, rho.v[0], rho.v[1], rho.v[2], rho.v[3]), 8 * l0};
     p1 = Slot{pick4f(l1, oxs[0], oxs[1], oxs[2], oxs[3]), pick4f(l1, oys[0], oys[1], oys[2], oys[3]),
               pick4f(l1, zs[0], zs[1], zs[2], zs[3]),
-              pick4f(l1, rho.v[0], rho.v[1], rho.v[2], rho.v[3]), l1};
+              pick4f(l1, rho.v[0], rho.v[1], rho.v[2], rho.v[3]), 8 * l1};
     p2 = Slot{pick4f(l2, oxs[0], oxs[1], oxs[2], oxs[3]), pick4f(l2, oys[0], oys[1], oys[2], oys[3]),
               pick4f(l2, zs[0], zs[1], zs[2], zs[3]),
-              pick4f(l2, rho.v[0], rho.v[1], rho.v[2], rho.v[3]), l2};
+              pick4f(l2, rho.v[0], rho.v[1], rho.v[2], rho.v[3]), 8 * l2};
     e0 = fabsf(side32(p0.ox, p0.oy, p1.ox, p1.oy));
     e1 = fabsf(side32(p1.ox, p1.oy, p2.ox, p2.oy));
     e2 = fabsf(side32(p2.ox, p2.oy, p0.ox, p0.oy));
@@ -308,38 +321,36 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
 
   // ---- the walk: leave t through face f (the carried face).  The next
   // face record is requested as soon as f is known, so its load overlaps
-  // the face update and the exit crossing.
-  double acc = 0.0;
-  int steps = 1;
-  bool last = false;
-  FaceRec R = load_facerec(frec, 4 * (int64_t)t + f);
+  // the face update and the exit crossing.  Each step has one exit, at its
+  // top: the hull, a record without back link, the step bound, or a
+  // hand-over decided in the step before (steps = -1).
+  double acc = 0.0;  // sum of (rho_in + rho_out) dz: twice the integral
+  int steps = 1;     // tets entered
+  FaceRec R = load_facerec<kWide>(frec, t, f8);
   for (;;) {
     // exit crossing in FP32, height above the entry crossing (does not wait
     // for R)
     const float den = (e0 + e1) + e2;
-    const float M = fmaxf(absmax3(p0.ox, p0.oy, p1.ox), absmax3(p1.oy, p2.ox, p2.oy));
+    const float M = amax(absmax3(p0.ox, p0.oy, p1.ox), absmax3(p1.oy, p2.ox, p2.oy));
     const float inv = rcp32(den);
     float dz = fmaf(e1, p0.z, fmaf(e2, p1.z, e0 * p2.z)) * inv;
     float r_out = fmaf(e1, p0.rho, fmaf(e2, p1.rho, e0 * p2.rho)) * inv;
-    if (R.nb < 0) {  // leaves the hull: last crossing below
-      last = true;
-      break;
-    }
-    if (R.code & kFaceRecBad) break;
-    if (fabsf(den) < flt.k_ill * M * M) {
+    const bool stop = (int)R.code < 0 || (unsigned)steps >= (unsigned)g.max_steps;
+    if (!stop && fabsf(den) < flt.k_ill * M * M) {
       // rare: FP32 weights of a nearly vertical face are too uncertain
-      const Cross64 o = face_cross64(rec64, t, f, x, y, load_rho4(rec32, t));
+      const Cross64 o = face_cross64(rec64, t, f8 >> 3, x, y, load_rho4(rec32, t));
       dz = (float)(o.z - zc);
       r_out = o.rho;
     }
+    if (stop) break;
+    bool ok = true;
     if (kFullSlab) {
-      acc += (double)(0.5f * (rc + r_out) * dz);
+      acc += (double)((rc + r_out) * dz);
     } else {
       bool inside;
-      if (!slab_inside(rec64, rec32, t, enter, f, x, y, zext, zc, zc + (double)dz, g, flt,
-                       inside))
-        break;
-      if (inside) acc += (double)(0.5f * (rc + r_out) * dz);
+      ok = slab_inside(rec64, rec32, t, enter, f8 >> 3, x, y, zext, zc, zc + (double)dz, g, flt,
+                       inside);
+      if (inside) acc += (double)((rc + r_out) * dz);
     }
     zc += (double)dz;
     rc = r_out;
@@ -348,56 +359,48 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     p2.z -= dz;
 
     // enter the next tet through its face `back`; its fourth vertex Q
-    if (++steps > g.max_steps) {
-      cnt.step_cap++;
-      crossed = 0;
-      return true;
-    }
-    const int back = (int)(R.code & 3u);
-    const float zref = self(p0.lab == (f ^ 1), p0.z, self(p1.lab == (f ^ 1), p1.z, p2.z));
-    Slot qv{(float)(R.qx - x), (float)(R.qy - y), zref + R.dqz, R.rho, back};
+    const int back8 = (int)(R.code & 0x18u);
+    Slot qv{(float)(R.qx - x), (float)(R.qy - y), fmaf((p0.z + p1.z) + p2.z, 1.0f / 3.0f, R.dqz),
+            R.rho, back8};
     // labels of the carried vertices in the next tet
-    p0.lab = (int)((R.code >> (2 * (p0.lab ^ f))) & 3u);
-    p1.lab = (int)((R.code >> (2 * (p1.lab ^ f))) & 3u);
-    p2.lab = (int)((R.code >> (2 * (p2.lab ^ f))) & 3u);
+    p0.lab8 = (int)((R.code >> (p0.lab8 ^ f8)) & 31u);
+    p1.lab8 = (int)((R.code >> (p1.lab8 ^ f8)) & 31u);
+    p2.lab8 = (int)((R.code >> (p2.lab8 ^ f8)) & 31u);
     t = R.nb;
-    enter = back;
+    enter = back8 >> 3;
 
     // sides of the new vertex and the exit face
     float s0 = side32(p0.ox, p0.oy, qv.ox, qv.oy);
     float s1 = side32(p1.ox, p1.oy, qv.ox, qv.oy);
     float s2 = side32(p2.ox, p2.oy, qv.ox, qv.oy);
-    const float Mt = fmaxf(fmaxf(M, fabsf(qv.ox)), fabsf(qv.oy));
+    const float Mt = amax(amax(M, fabsf(qv.ox)), fabsf(qv.oy));
     const float bound = fmaf(flt.k_err, Mt * Mt, flt.thr);
+    bool zero = false;
     if (!(fminf(fminf(fabsf(s0), fabsf(s1)), fabsf(s2)) > bound)) {
       // rare: the FP64 walk's signs for the sides the filter leaves open
-      bool zero = false;
       if (!(fabsf(s0) > bound)) {
-        const int sg = sign64(rec64, t, p0.lab, back, x, y, g.lz);
+        const int sg = sign64(rec64, t, p0.lab8 >> 3, enter, x, y, g.lz);
         zero = zero || sg == 0;
         s0 = copysignf(fabsf(s0), (float)sg);
       }
       if (!(fabsf(s1) > bound)) {
-        const int sg = sign64(rec64, t, p1.lab, back, x, y, g.lz);
+        const int sg = sign64(rec64, t, p1.lab8 >> 3, enter, x, y, g.lz);
         zero = zero || sg == 0;
         s1 = copysignf(fabsf(s1), (float)sg);
       }
       if (!(fabsf(s2) > bound)) {
-        const int sg = sign64(rec64, t, p2.lab, back, x, y, g.lz);
+        const int sg = sign64(rec64, t, p2.lab8 >> 3, enter, x, y, g.lz);
         zero = zero || sg == 0;
         s2 = copysignf(fabsf(s2), (float)sg);
       }
-      if (zero) break;
     }
     // the face replacing P_j: j = 2 if s0 < 0 < s1, 0 if s1 < 0 < s2, 1 if s2 < 0 < s0
     const bool n0 = __builtin_signbit(s0), n1 = __builtin_signbit(s1), n2 = __builtin_signbit(s2);
     const bool r2 = n0 && !n1, r0 = n1 && !n2, r1 = n2 && !n0;
-    if (!(r0 || r1 || r2)) break;  // (with exact signs exactly one holds)
-    f = seli(r0, p0.lab, seli(r1, p1.lab, p2.lab));
-    R = load_facerec(frec, 4 * (int64_t)t + f);
+    f8 = seli(r0, p0.lab8, seli(r1, p1.lab8, p2.lab8));
+    R = load_facerec<kWide>(frec, t, f8);
     if (!kFullSlab)
-      zext = fmaxf(fmaxf(p0.z, p1.z), fmaxf(p2.z, qv.z)) -
-             fminf(fminf(p0.z, p1.z), fminf(p2.z, qv.z));
+      zext = amax(amax(p0.z, p1.z), amax(p2.z, qv.z)) - fminf(fminf(p0.z, p1.z), fminf(p2.z, qv.z));
     // new face: P_j := Q; sides (P_{j-1}, Q) = s_{j-1}, (Q, P_{j+1}) = -s_{j+1}
     const float ne0 = self(r1, s0, self(r0, -s1, e0));
     const float ne1 = self(r2, s1, self(r1, -s2, e1));
@@ -408,20 +411,28 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     p0 = sel_slot(r0, qv, p0);
     p1 = sel_slot(r1, qv, p1);
     p2 = sel_slot(r2, qv, p2);
+    // with exact signs exactly one of r0, r1, r2 holds
+    const bool fail = zero || !(r0 || r1 || r2) || !ok;
+    steps = fail ? -1 : steps + 1;
   }
-  if (!last) return false;
-  // the last crossing in FP64, where the lanes of a wave are together again
-  const Cross64 out = face_cross64(rec64, t, f, x, y, load_rho4(rec32, t));
-  const double dz = out.z - zc;
-  bool inside = true;
-  if (!kFullSlab) {
-    const float zext = (float)(fmax(fmax(p0.z, p1.z), p2.z) - fmin(fmin(p0.z, p1.z), p2.z));
-    if (!slab_inside(rec64, rec32, t, enter, f, x, y, zext, zc, out.z, g, flt, inside))
-      return false;
+  if (steps < 0) return false;
+  if ((int)R.code < 0) {
+    if (R.nb >= 0) return false;  // the mesh lacks the back link
+    // the last crossing in FP64, where the lanes of a wave are together again
+    const Cross64 out = face_cross64(rec64, t, f8 >> 3, x, y, load_rho4(rec32, t));
+    const double dz = out.z - zc;
+    bool inside = true;
+    if (!kFullSlab) {
+      const float ze = amax(amax(p0.z, p1.z), p2.z) - fminf(fminf(p0.z, p1.z), p2.z);
+      if (!slab_inside(rec64, rec32, t, enter, f8 >> 3, x, y, ze, zc, out.z, g, flt, inside))
+        return false;
+    }
+    if (inside) acc += (double)(rc + out.rho) * dz;
+    value = 0.5 * acc;
+    crossed = steps;
+    return true;
   }
-  if (inside) acc += (double)(0.5f * (rc + out.rho)) * dz;
-  value = acc;
-  crossed = steps;
+  cnt.step_cap++;  // the step bound
   return true;
 }
 

@@ -30,7 +30,8 @@ static int check_faces() {
 int main(int argc, char **argv) {
   if (argc == 2 && std::string(argv[1]) == "faces") return check_faces();
   if (argc != 3 && argc != 4) return 2;
-  const bool mixed = argc == 4;
+  const bool mixed = argc == 4;  // "mixed": 32-bit face-record offsets, "mixedwide": 64-bit
+  const bool wide = mixed && std::string(argv[3]) == "mixedwide";
   FILE *f = fopen(argv[1], "rb");
   if (!f) return 3;
   long long nt = 0, n = 0;
@@ -92,12 +93,21 @@ int main(int argc, char **argv) {
     const uint64_t key = sdtfe::sample_key(seed, pix[i], (uint32_t)smp[i]);
     double v = 0.0;
     // the mixed walk hands some rays to the FP64 walk, as the map kernels do
-    const bool done =
-        mixed && (mc.flt.full_slab
-                      ? sdtfe::walk_mixed<true>(rec32.data(), rec.data(), frec.data(), faces, (int)ent[i], x[i],
-                                                y[i], g, mc.flt, v, crossed, c)
-                      : sdtfe::walk_mixed<false>(rec32.data(), rec.data(), frec.data(), faces, (int)ent[i], x[i],
-                                                 y[i], g, mc.flt, v, crossed, c));
+    bool done = false;
+    if (mixed) {
+      const auto *r32 = rec32.data();
+      const auto *r64 = rec.data();
+      const auto *fr = frec.data();
+      const int e = (int)ent[i];
+      if (wide)
+        done = mc.flt.full_slab
+                   ? sdtfe::walk_mixed<true, true>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, v, crossed, c)
+                   : sdtfe::walk_mixed<false, true>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, v, crossed, c);
+      else
+        done = mc.flt.full_slab
+                   ? sdtfe::walk_mixed<true, false>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, v, crossed, c)
+                   : sdtfe::walk_mixed<false, false>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, v, crossed, c);
+    }
     if (!done) {
       if (mixed) c.fp64_redo++;
       v = sdtfe::walk64(rec.data(), (int)ent[i], x[i], y[i], g, key, crossed, c);

@@ -53,13 +53,16 @@ def records32(pos, tetra, rho):
 
 def face_records(pos, tetra, rho):
     """The mixed walk's 32-byte face records (4 per tet): the new vertex
-    (x, y FP64), the tet across face f, the face's index there plus the
-    labels there of this tet's vertices f^1, f^2, f^3, and the new vertex's
-    z relative to vertex f^1 and density (FP32)."""
+    (x, y FP64), the tet across face f, a code (8 x the face's index there in
+    bits 3-4, 8 x the labels there of this tet's vertices f^k in bits
+    8k..8k+4, bit 31 when the walk stops: hull or missing back link), and
+    the new vertex's z relative to the mean z of face f's vertices and its
+    density (FP32)."""
     t = tetra.shape[0]
     rec = np.zeros((t, 4), dtype=np.dtype([("qx", "<f8"), ("qy", "<f8"), ("nb", "<i4"),
                                            ("code", "<u4"), ("dqz", "<f4"), ("rho", "<f4")]))
     assert rec.dtype.itemsize == 32
+    stop = np.int64(1) << 31
     verts, nbrs = tetra[:, :4], tetra[:, 4:]
     for f in range(4):
         n = nbrs[:, f]
@@ -69,25 +72,26 @@ def face_records(pos, tetra, rho):
         back = np.full(t, -1)
         for k in range(3, -1, -1):
             back = np.where(nbrs[nn, k] == np.arange(t), k, back)
-        code = np.where(back < 0, 1 << 8, back & 3).astype(np.int64)
+        code = np.where(back < 0, stop, (back & 3) << 3).astype(np.int64)
+        zs = np.zeros(t)
         for k in (1, 2, 3):
             v = verts[:, f ^ k]
+            zs = zs + pos[v, 2]
             lab = np.full(t, -1)
             for m in range(3, -1, -1):
                 lab = np.where(verts[nn, m] == v, m, lab)
-            code |= np.where((lab < 0) | (lab == back), 1 << 8, 0)
-            code |= (lab & 3) << (2 * k)
+            code |= np.where((lab < 0) | (lab == back), stop, 0)
+            code |= (lab & 3) << (8 * k + 3)
         q = verts[nn, back & 3]
-        ref = verts[:, f ^ 1]
-        rec["code"][:, f] = np.where(ok, code, 0)
+        rec["code"][:, f] = np.where(ok, code, stop)
         rec["qx"][:, f] = np.where(ok, pos[q, 0], 0)
         rec["qy"][:, f] = np.where(ok, pos[q, 1], 0)
-        rec["dqz"][:, f] = np.where(ok, (pos[q, 2] - pos[ref, 2]).astype(np.float32), 0)
+        rec["dqz"][:, f] = np.where(ok, (pos[q, 2] - zs / 3.0).astype(np.float32), 0)
         rec["rho"][:, f] = np.where(ok, rho[q].astype(np.float32), 0)
     return rec.reshape(-1)
 
 
-def run_walk(host_walk, tmp_path, g, mixed=False):
+def run_walk(host_walk, tmp_path, g, mixed=False, wide=False):
     rec = records(g["pos"], g["tetra"], g["rho"], g["nabla"])
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
@@ -107,7 +111,8 @@ def run_walk(host_walk, tmp_path, g, mixed=False):
             face_records(g["pos"], g["tetra"], g["rho"]).tofile(f)
             zr = (lo[2], hi[2])
             np.array([*org, float((0.5 * (hi - lo)).max()), *zr]).tofile(f)
-    r = subprocess.run([host_walk, str(inp), str(out)] + (["mixed"] if mixed else []),
+    mode = (["mixedwide" if wide else "mixed"] if mixed else [])
+    r = subprocess.run([host_walk, str(inp), str(out)] + mode,
                        check=True, capture_output=True, text=True)
     res = np.fromfile(out).reshape(-1, 2)
     return res, r.stderr
@@ -127,8 +132,9 @@ def test_face_table(host_walk):
     assert r.returncode == 0, r.stderr
 
 
+@pytest.mark.parametrize("wide", [False, True], ids=["narrow", "wide"])
 @pytest.mark.parametrize("slab", ["hull", "cut"])
-def test_device_mixed_walk_code(golden, host_walk, tmp_path, slab):
+def test_device_mixed_walk_code(golden, host_walk, tmp_path, slab, wide):
     """Mixed walk: same tets as the reference walk, values within 1e-5.
     'hull': the golden rays, whose slab holds the hull's z range (midpoints
     are not tested); 'cut': the same rays through the slab 0.25 < z < 0.75,
@@ -140,7 +146,7 @@ def test_device_mixed_walk_code(golden, host_walk, tmp_path, slab):
         ref, crossed = want[:, 0], want[:, 1].astype(np.int32)
     else:
         ref, crossed = g["value"], g["crossed"]
-    res, err = run_walk(host_walk, tmp_path, g, mixed=True)
+    res, err = run_walk(host_walk, tmp_path, g, mixed=True, wide=wide)
     assert f"full_slab {int(slab == 'hull')}" in err, err
     np.testing.assert_array_equal(res[:, 1].astype(np.int32), crossed)
     v = res[:, 0]
