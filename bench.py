@@ -182,7 +182,8 @@ def bench_cutouts(a):
     in one persistent launch (sdtfe_batch) against the same maps one launch
     at a time.  Cutout i: particles log-uniform in [1e3, 2e5], grid
     log-uniform in [64, 1024] (multiple of 4), 4-16 samples per pixel."""
-    from sdtfe_amd import capi, serve
+    import collections
+    from sdtfe_amd import autotune, capi, serve
     prec = _precision(a)
     t0 = time.perf_counter()
     reqs = serve.make_requests(a.cutouts, seed=a.seed, workers=16)
@@ -191,11 +192,18 @@ def bench_cutouts(a):
     tess = [capi.Tessellation(r.pd, r.tetra) for r in reqs]
     bufs = [capi.DeviceBuffer(r.grid * r.grid * 8) for r in reqs]
     params = [t.params(r.grid, r.samples, seed=r.seed, precision=prec) for t, r in zip(tess, reqs)]
-    lpp = [4 if r.grid <= 256 else 1 for r in reqs]
     rays = sum(r.rays for r in reqs)
     for t in tess:
         t.sync()
     with capi.Batch() as b:
+        # lanes per pixel (rays per wave from one pixel) per size class of
+        # map, timed once per class before the timed region
+        t0 = time.perf_counter()
+        tuner = autotune.Autotuner()
+        lpp = [tuner.lanes_for(b, t, p) for t, p in zip(tess, params)]
+        tune_s = time.perf_counter() - t0
+        log(f"[bench] autotuned {len(tuner.table)} size classes in {tune_s:.1f} s: lanes per "
+            f"pixel {dict(sorted(collections.Counter(lpp).items()))}")
         items = list(zip(tess, params, bufs))
         for _ in range(a.warmup):
             b.run(items, lanes_per_pixel=lpp)
@@ -227,6 +235,8 @@ def bench_cutouts(a):
            "config": {"workload": f"{len(reqs)} halo cutouts (1e3-2e5 particles, 64-1024 px, "
                                   f"4-16 samples/pixel) in one batched launch",
                       "sample_rays": rays, "maps": len(reqs)},
+           "lanes_per_pixel": {str(k): v for k, v in sorted(collections.Counter(lpp).items())},
+           "autotune_s": round(tune_s, 2),
            "batched_walk_ms": round(float(np.mean(walk)), 3),
            "one_launch_per_map_s": round(single_s, 3),
            "batched_over_one_by_one": round(single_s / (ms / 1e3), 2)}

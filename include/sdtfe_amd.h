@@ -191,7 +191,8 @@ typedef struct {
   int64_t hbm_budget_bytes;
   int max_active;
   int tile_budget;     /* 256-lane tiles per launch */
-  int lanes_per_pixel; /* 1, 2, 4, 8, 16; <= 0: 4 for maps up to 256^2, else 1 */
+  int lanes_per_pixel; /* 1, 2, 4, 8, 16; <= 0: by samples per pixel S (16 for
+                          16 <= S <= 256, 4 for 4 <= S < 16, else 1) */
   int precision;       /* SDTFE_PREC_*, for every request */
 } sdtfe_server_config;
 typedef struct {

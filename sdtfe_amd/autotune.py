@@ -13,7 +13,7 @@ import os
 
 from . import capi
 
-CANDIDATES = (1, 2, 4, 8)
+CANDIDATES = (1, 2, 4, 8, 16)
 
 
 def size_class(grid: int, samples: int, n_tetra: int) -> str:

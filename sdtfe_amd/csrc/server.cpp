@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "sdtfe_host.h"
+#include "walk_common.h"
 
 namespace sdtfe {
 namespace {
@@ -153,7 +154,7 @@ void setup_loop(sdtfe_server *sv) {
     if (rc == 0 && hipMemset(r->d_map, 0, sizeof(double) * npix) != hipSuccess) rc = SDTFE_EHIP;
     if (rc == 0) {
       r->lpp = sv->cfg.lanes_per_pixel > 0 ? sv->cfg.lanes_per_pixel
-                                           : (r->params.grid_dim <= 256 ? 4 : 1);
+                                           : map_lanes(r->params.n_mc_samp);
       r->n_tiles = batch_tiles(r->params, r->lpp);
     }
     {

@@ -89,4 +89,32 @@ __device__ inline void lane_pixel(int grid, int &px, int &py) {
   py = by * 16 + (w >> 1) * 8 + (lane >> 3);
 }
 
+// Wave tile of L lanes per pixel (1, 2, 4, 8, 16): 64 / L pixels as
+// wave_bw x wave_bh, the L lanes of a pixel adjacent.
+SDTFE_HD constexpr int wave_bw(int L) { return L <= 2 ? 8 : L <= 8 ? 4 : 2; }
+SDTFE_HD constexpr int wave_bh(int L) { return L == 1 ? 8 : L <= 4 ? 4 : 2; }
+
+// Lanes per pixel of a single map: the rays of a wave then start together
+// on a 2x2 (L = 16) or 4x4 (L = 4) pixel patch and share more records.
+// The mixed walk's hand-over window bounds the samples a split pixel may
+// have (kDeferWindow).
+SDTFE_HD constexpr int map_lanes(int samples) {
+  return samples > kDeferWindow ? 1 : samples >= 16 ? 16 : samples >= 4 ? 4 : 1;
+}
+
+// Pixel and sample slice j (0..L-1) of a lane with L lanes per pixel:
+// blocks of 256 lanes cover 2 x 2 wave tiles, remapped as in lane_pixel.
+template <int L>
+__device__ inline void split_pixel(int grid, int &px, int &py, int &j) {
+  constexpr int bw = wave_bw(L), bh = wave_bh(L);
+  const int nbx = (grid + 2 * bw - 1) / (2 * bw);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = tile % nbx, by = tile / nbx;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = lane / L;
+  j = lane % L;
+  px = bx * 2 * bw + (w & 1) * bw + q % bw;
+  py = by * 2 * bh + (w >> 1) * bh + q / bw;
+}
+
 }  // namespace sdtfe

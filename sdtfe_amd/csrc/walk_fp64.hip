@@ -7,47 +7,75 @@
 //
 // Layout: each step of a walk reads one 144 B record (sdtfe::Rec64) that
 // holds the tet's neighbours, its four vertices and its field, so a step is a
-// single dependent load.  A map launch gives each lane one pixel (8x8 pixels
-// per wave, 16x16 per block) and walks its samples in order, which keeps the
-// 64 rays of a wave in the same column of tets.
+// single dependent load.  A map launch spreads each pixel's samples over 4
+// lanes (4 or more samples), so the 64 rays of a wave start together on a
+// 4x4-pixel patch of the map and cross mostly the same tets.
 #include "sdtfe_host.h"
 #include "walk_fp64_core.h"
 
 namespace sdtfe {
 
-// One pixel per lane: S samples in order, pixel value = sum / n_avg
-// (src/dtfe.c:613-645).  accumulate: out += value, else out = value.
-__global__ __launch_bounds__(256) void map_fp64_kernel(const Rec64 *__restrict__ rec,
+// L lanes per pixel (lane j walks samples j, j + L, ...; L = 1: one pixel
+// per lane, S samples in order).  The pixel's first lane takes the values in
+// sample order after each round, so pixel value = sum / n_avg is the same
+// for every L (src/dtfe.c:613-645).  accumulate: out += value, else out =
+// value.
+template <int L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void map_fp64_kernel(const Rec64 *__restrict__ rec,
                                                       EntryGrid eg, MapGeom g, double *out,
                                                       int *navg, int accumulate,
                                                       unsigned long long *stats) {
-  int px, py;
-  lane_pixel(g.grid, px, py);
+  int px, py, j;
+  split_pixel<L>(g.grid, px, py, j);
+  const bool active = px < g.grid && py < g.grid;
+  const uint64_t pix = (uint64_t)py * (uint64_t)g.grid + (uint64_t)px;
   LaneCounters cnt;
-  if (px < g.grid && py < g.grid) {
-    const uint64_t pix = (uint64_t)py * (uint64_t)g.grid + (uint64_t)px;
-    double sum = 0.0;
-    int n_avg = g.samples;
-    for (int s = 0; s < g.samples; ++s) {
-      const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
-      double xt, yt;
-      sample_point(g, px, py, key, xt, yt);
-      cnt.samples++;
-      const int entry = locate_entry(eg, xt, yt);
-      if (entry < 0) {
-        cnt.missed++;
-        --n_avg;
-        continue;
+  double sum = 0.0;
+  int n_avg = g.samples;
+  // one sample: 0 a miss, 1 a walk that crossed tets, 2 one that crossed
+  // none (both add v, src/dtfe.c:630-636); -1 not walked
+  auto sample = [&](int s, double &v) -> int {
+    const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
+    double xt, yt;
+    sample_point(g, px, py, key, xt, yt);
+    cnt.samples++;
+    v = 0.0;
+    const int entry = locate_entry(eg, xt, yt);
+    if (entry < 0) {
+      cnt.missed++;
+      return 0;
+    }
+    int crossed;
+    v = walk64(rec, entry, xt, yt, g, key, crossed, cnt);
+    cnt.tet_steps += (unsigned)crossed;
+    if (crossed == 0) cnt.failed++;
+    return crossed > 0 ? 1 : 2;
+  };
+  auto take = [&](int k, double v) {
+    if (k >= 1) sum += v;
+    if (k == 0 || k == 2) --n_avg;
+  };
+  if (L == 1) {
+    if (active)
+      for (int s = 0; s < g.samples; ++s) {
+        double v;
+        const int k = sample(s, v);
+        take(k, v);
       }
-      int crossed;
-      const double v = walk64(rec, entry, xt, yt, g, key, crossed, cnt);
-      sum += v;
-      cnt.tet_steps += (unsigned)crossed;
-      if (crossed == 0) {
-        cnt.failed++;
-        --n_avg;
+  } else {
+    const int lead = (int)(threadIdx.x & 63) - j;
+    for (int s0 = 0; s0 < g.samples; s0 += L) {
+      double v = 0.0;
+      int k = -1;
+      if (active && s0 + j < g.samples) k = sample(s0 + j, v);
+      for (int i = 0; i < L; ++i) {
+        const double vi = __shfl(v, lead + i, 64);
+        const int ki = __shfl(k, lead + i, 64);
+        if (j == 0) take(ki, vi);
       }
     }
+  }
+  if (active && j == 0) {
     const double val = n_avg > 0 ? sum / n_avg : 0.0;
     if (accumulate)
       out[pix] += val;
@@ -82,11 +110,22 @@ __global__ void locate_kernel(EntryGrid eg, int n, const double *x, const double
 
 // ------------------------------------------------------------ launchers
 
+template <int L>
+void launch_map_fp64_l(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
+  constexpr int tw = 2 * wave_bw(L), th = 2 * wave_bh(L);
+  const int nb = ((g.grid + tw - 1) / tw) * ((g.grid + th - 1) / th);
+  map_fp64_kernel<L><<<nb, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg, accumulate,
+                                                t->stats);
+}
+
 void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
-  const int nb = (g.grid + 15) / 16;
   (void)hipEventRecord(t->evk0, t->stream);
-  map_fp64_kernel<<<nb * nb, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg, accumulate,
-                                                  t->stats);
+  // at most 4 lanes per pixel: the FP64 walk's registers allow 3 waves per
+  // SIMD, and 16 lanes per pixel would need more
+  if (map_lanes(g.samples) >= 4)
+    launch_map_fp64_l<4>(t, g, out, navg, accumulate);
+  else
+    launch_map_fp64_l<1>(t, g, out, navg, accumulate);
   (void)hipEventRecord(t->evk1, t->stream);
 }
 

@@ -14,44 +14,76 @@ __device__ inline void load_faces(uint8_t *faces) {
 
 __global__ void defer_reset_kernel(int *count) { *count = 0; }
 
-template <bool kFullSlab, bool kWide>
+// First pass.  L lanes per pixel: lane j of a pixel walks its samples j,
+// j + L, ...; the L lanes of a wave's pixel walk in step (a wave's rays
+// start together at the top of one small patch of the map, so they share
+// more face records), and after each round the pixel's first lane takes the
+// L values in sample order, so the map does not depend on L.  With L > 1
+// the samples must fit the window of handed-over samples (kDeferWindow).
+template <bool kFullSlab, bool kWide, int L>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDTFE_MIXED_WAVES))) void
-map_mixed_kernel(const Rec32 *__restrict__ rec32,
-                                                       const Rec64 *__restrict__ rec64,
-                                                       const FaceRec *__restrict__ frec,
-                                                       EntryGrid eg, MapGeom g, MixedConsts mc,
-                                                       double *out, int *navg, int accumulate,
-                                                       DeferQueue dq,
-                                                       unsigned long long *stats) {
+map_mixed_kernel(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec64,
+                 const FaceRec *__restrict__ frec, EntryGrid eg, MapGeom g, MixedConsts mc,
+                 double *out, int *navg, int accumulate, DeferQueue dq,
+                 unsigned long long *stats) {
   __shared__ uint8_t faces[kFaceCodes];
   load_faces(faces);
-  int px, py;
-  lane_pixel(g.grid, px, py);
+  int px, py, j;
+  split_pixel<L>(g.grid, px, py, j);
+  const bool active = px < g.grid && py < g.grid;
+  const uint64_t pix = (uint64_t)py * (uint64_t)g.grid + (uint64_t)px;
   LaneCounters cnt;
-  if (px < g.grid && py < g.grid) {
-    const uint64_t pix = (uint64_t)py * (uint64_t)g.grid + (uint64_t)px;
-    PixelSamples<1> ps(g.samples);
-    for (int s = 0; s < g.samples; ++s) {
-      const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
-      double xt, yt;
-      sample_point(g, px, py, key, xt, yt);
-      cnt.samples++;
-      const int entry = locate_entry(eg, xt, yt);
-      if (entry < 0) {
-        cnt.missed++;
-        continue;
+  PixelSamples<L == 1 ? 1 : 4> ps(g.samples);  // the pixel's first lane's
+  // one sample: 1 walked (value v, crossed > 0), 0 adds nothing (a miss, a
+  // failed walk), 2 handed over; -1 not walked
+  auto sample = [&](int s, double &v) -> int {
+    const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
+    double xt, yt;
+    sample_point(g, px, py, key, xt, yt);
+    cnt.samples++;
+    v = 0.0;
+    const int entry = locate_entry(eg, xt, yt);
+    if (entry < 0) {
+      cnt.missed++;
+      return 0;
+    }
+    int crossed;
+    if (!walk_mixed<kFullSlab, kWide>(rec32, rec64, frec, faces, entry, xt, yt, g, mc.flt, v,
+                                      crossed, cnt))
+      return 2;
+    cnt.tet_steps += (unsigned)crossed;
+    if (crossed == 0) cnt.failed++;
+    return crossed > 0 ? 1 : 0;
+  };
+  if (L == 1) {
+    if (active)
+      for (int s = 0; s < g.samples; ++s) {
+        double v;
+        const int k = sample(s, v);
+        if (k == 1)
+          ps.add(v, 1);
+        else if (k == 2 && !ps.hand_over(s))
+          break;  // s and every later sample go to the FP64 walk
       }
-      double v;
-      int crossed;
-      if (walk_mixed<kFullSlab, kWide>(rec32, rec64, frec, faces, entry, xt, yt, g, mc.flt, v,
-                                       crossed, cnt)) {
-        ps.add(v, crossed);
-        cnt.tet_steps += (unsigned)crossed;
-        if (crossed == 0) cnt.failed++;
-      } else if (!ps.hand_over(s)) {
-        break;
+  } else {
+    const int lead = (int)(threadIdx.x & 63) - j;
+    for (int s0 = 0; s0 < g.samples; s0 += L) {
+      double v = 0.0;
+      int k = -1;
+      if (active && s0 + j < g.samples) k = sample(s0 + j, v);
+      for (int i = 0; i < L; ++i) {
+        const double vi = __shfl(v, lead + i, 64);
+        const int ki = __shfl(k, lead + i, 64);
+        if (j == 0) {
+          if (ki == 1)
+            ps.add(vi, 1);
+          else if (ki == 2)
+            ps.hand_over(s0 + i);  // within the window: samples <= kDeferWindow
+        }
       }
     }
+  }
+  if (active && j == 0) {
     if (ps.clean(g.samples)) {
       const double val = ps.n_ok > 0 ? ps.sum / ps.n_ok : 0.0;
       if (accumulate)
@@ -144,6 +176,35 @@ int ensure_defer(sdtfe_tess *t, size_t entries) {
 
 }  // namespace
 
+template <int L>
+void launch_first_pass_l(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, double *out,
+                         int *navg, int accumulate, const DeferQueue &dq) {
+  constexpr int tw = 2 * wave_bw(L), th = 2 * wave_bh(L);
+  const int nb = ((g.grid + tw - 1) / tw) * ((g.grid + th - 1) / th);
+  const bool wide = (int64_t)t->nt > kFaceRecNarrowTets;
+  if (mc.flt.full_slab && !wide)
+    map_mixed_kernel<true, false, L><<<nb, 256, 0, t->stream>>>(
+        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+  else if (!wide)
+    map_mixed_kernel<false, false, L><<<nb, 256, 0, t->stream>>>(
+        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+  else if (mc.flt.full_slab)
+    map_mixed_kernel<true, true, L><<<nb, 256, 0, t->stream>>>(
+        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+  else
+    map_mixed_kernel<false, true, L><<<nb, 256, 0, t->stream>>>(
+        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+}
+
+void launch_first_pass(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, double *out,
+                       int *navg, int accumulate, const DeferQueue &dq) {
+  switch (map_lanes(g.samples)) {
+    case 16: launch_first_pass_l<16>(t, g, mc, out, navg, accumulate, dq); break;
+    case 4: launch_first_pass_l<4>(t, g, mc, out, navg, accumulate, dq); break;
+    default: launch_first_pass_l<1>(t, g, mc, out, navg, accumulate, dq); break;
+  }
+}
+
 // The queue holds a quarter of the pixels; a map that overflows it is run
 // again with room for all of them (an accumulating map gets that room from
 // the start, since its first pass cannot be repeated).
@@ -154,24 +215,11 @@ int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
   int rc = ensure_defer(t, want);
   if (rc) return rc;
   const MixedConsts mc = mixed_consts(t, g);
-  const int nb = (g.grid + 15) / 16;
   for (;;) {
     const DeferQueue dq{t->defer_buf, t->defer_count, (int)std::min(t->defer_cap, npix)};
     defer_reset_kernel<<<1, 1, 0, t->stream>>>(t->defer_count);
     SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
-    const bool wide = (int64_t)t->nt > kFaceRecNarrowTets;
-    if (mc.flt.full_slab && !wide)
-      map_mixed_kernel<true, false><<<nb * nb, 256, 0, t->stream>>>(
-          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
-    else if (!wide)
-      map_mixed_kernel<false, false><<<nb * nb, 256, 0, t->stream>>>(
-          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
-    else if (mc.flt.full_slab)
-      map_mixed_kernel<true, true><<<nb * nb, 256, 0, t->stream>>>(
-          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
-    else
-      map_mixed_kernel<false, true><<<nb * nb, 256, 0, t->stream>>>(
-          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+    launch_first_pass(t, g, mc, out, navg, accumulate, dq);
     SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
     map_deferred_kernel<<<1024, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg,
                                                      accumulate, dq, t->stats);

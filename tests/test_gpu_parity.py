@@ -187,3 +187,29 @@ def test_errors_are_reported():
             t.map(0, 4)
         rho, st = t.map(16, 2, box_len=3.0)  # most samples outside the hull's shadow
         assert st["missed"] > 0 and np.isfinite(rho).all()
+
+
+@pytest.mark.parametrize("S", [1, 5, 20, 300])
+def test_sample_split_maps_match_oracle(oracle, S):
+    """Maps whose samples are spread over 1, 4 or 16 lanes per pixel (by
+    the sample count; 20 leaves the second round partly empty, 300 exceeds
+    the split's window and runs one lane per pixel): FP64 against the oracle,
+    mixed against FP64 with the same tets."""
+    _say(f"[gpu] clustered-8192 48x48x{S}, lanes per pixel by sample count")
+    pos = workloads.clustered(8192, seed=4)
+    tetra = triangulate(pos)
+    pd = to_particle_data(pos)
+    G, seed = 48, 5
+    ref, st_o, navg_o = oracle.compute_density(pd.copy(), tetra, G, 1.0, 1.0, 1.0,
+                                               (0.5, 0.5, 0.5), S, 0.0, seed, want_navg=True)
+    with capi.Tessellation(pd, tetra) as t:
+        rho, st, navg = t.map(G, S, want_navg=True, seed=seed)
+        r32, s32, n32 = t.map(G, S, want_navg=True, seed=seed, precision=capi.PREC_MIXED)
+    np.testing.assert_array_equal(navg, navg_o)
+    assert _rel(rho, ref) <= RTOL_FP64
+    assert st["tet_steps"] == st_o["tet_steps"]
+    assert st["samples"] == G * G * S
+    np.testing.assert_array_equal(n32, navg)
+    assert s32["tet_steps"] == st["tet_steps"]
+    assert s32["samples"] == G * G * S
+    assert _rel(r32, rho) <= 1e-5
