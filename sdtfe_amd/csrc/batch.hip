@@ -97,11 +97,12 @@ __device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int
     return true;
   }
   if (kMixed) {
+    Resume rs;  // the second pass walks the sample again and resumes there
     const bool done = it.mc.flt.full_slab
                           ? walk_mixed<true, false>(it.rec32, it.rec64, it.facerec, faces, entry,
-                                                    xt, yt, g, it.mc.flt, v, crossed, cnt)
+                                                    xt, yt, g, it.mc.flt, v, crossed, cnt, rs)
                           : walk_mixed<false, false>(it.rec32, it.rec64, it.facerec, faces, entry,
-                                                     xt, yt, g, it.mc.flt, v, crossed, cnt);
+                                                     xt, yt, g, it.mc.flt, v, crossed, cnt, rs);
     if (!done) return false;
   } else {
     v = walk64(it.rec64, entry, xt, yt, g, key, crossed, cnt);
@@ -243,6 +244,9 @@ __global__ __launch_bounds__(256) void batch_deferred_kernel(const BatchItem *__
                                                             const BatchHdr *__restrict__ hdr,
                                                             const int *counters,
                                                             unsigned long long *item_stats) {
+  __shared__ uint8_t faces[kFaceCodes];
+  if (threadIdx.x < kFaceCodes) faces[threadIdx.x] = (uint8_t)face_code((int)threadIdx.x);
+  __syncthreads();
   const DeferQueue dq = hdr->dq;
   const int n = min(counters[kXcd], dq.cap);
   const int waves = gridDim.x * (blockDim.x >> 6);
@@ -251,7 +255,8 @@ __global__ __launch_bounds__(256) void batch_deferred_kernel(const BatchItem *__
     const BatchItem &it = items[d->item];
     LaneCounters cnt;
     int n_ok;
-    const double sum = finish_deferred_wave(d, it.rec64, it.grid, it.geom, n_ok, cnt);
+    const DeferCtx c{it.rec32, it.rec64, it.facerec, faces, it.grid, it.geom, it.mc.flt};
+    const double sum = finish_deferred_wave<false>(d, c, n_ok, cnt);
     if ((threadIdx.x & 63) == 0) it.out[d->pix] += n_ok > 0 ? sum / n_ok : 0.0;
     cnt.flush(item_stats + (size_t)d->item * ST_COUNT);  // the wave's lanes share the item
   }

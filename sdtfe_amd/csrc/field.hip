@@ -255,7 +255,7 @@ __global__ void face_record_kernel(const double4 *pos, const int *tets, int nt, 
   FaceRec r;
   r.nb = tets[8 * (int64_t)t + 4 + f];
   r.code = kFaceStop;
-  r.qx = r.qy = 0.0;
+  r.qxh = r.qxl = r.qyh = r.qyl = 0.f;
   r.dqz = r.rho = 0.f;
   if (r.nb >= 0) {
     const int n = r.nb;
@@ -275,8 +275,8 @@ __global__ void face_record_kernel(const double4 *pos, const int *tets, int nt, 
     }
     r.code = code;
     const double4 q = pos[tets[8 * (int64_t)n + (back & 3)]];
-    r.qx = q.x;
-    r.qy = q.y;
+    split32(q.x, r.qxh, r.qxl);
+    split32(q.y, r.qyh, r.qyl);
     r.dqz = (float)(q.z - zs / 3.0);
     r.rho = (float)q.w;
   }

@@ -46,8 +46,9 @@ static_assert(sizeof(Rec32) == 96, "Rec32 layout");
 // 16-byte loads: where a ray that leaves tet t through face f (the face
 // opposite vertex f) goes.  The walk carries the face's three vertices, so
 // the next tet only adds its fourth vertex, stored here.
-//   q     the new vertex: x, y in FP64; z minus the mean z of face f's three
-//         vertices ((z1 + z2) + z3) / 3 and the density, in FP32
+//   q     the new vertex: x and y each as an FP32 pair (hi = x rounded,
+//         lo = x - hi rounded), z minus the mean z of face f's three vertices
+//         ((z1 + z2) + z3) / 3 and the density, in FP32
 //   nb    tet across the face (-1 on the hull)
 //   code  8 x the face's index in nb (nb's vertex opposite it, i.e. the new
 //         vertex) in bits 3-4; 8 x the label in nb of t's vertex f^k in
@@ -56,7 +57,7 @@ static_assert(sizeof(Rec32) == 96, "Rec32 layout");
 //         at offset 8 (label ^ f); bit 31 (kFaceStop): the ray stops here
 //         (nb < 0, or the mesh lacks the back link)
 struct alignas(16) FaceRec {
-  double qx, qy;
+  float qxh, qxl, qyh, qyl;
   int nb;
   unsigned code;
   float dqz;

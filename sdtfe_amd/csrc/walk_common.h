@@ -89,6 +89,13 @@ __device__ inline void lane_pixel(int grid, int &px, int &py) {
   py = by * 16 + (w >> 1) * 8 + (lane >> 3);
 }
 
+// v as hi + lo in FP32: hi = v rounded, lo = (v - hi) rounded (v - hi is
+// exact in FP64), so hi + lo is within 2^-48 |v| of v.
+SDTFE_HD inline void split32(double v, float &hi, float &lo) {
+  hi = (float)v;
+  lo = (float)(v - (double)hi);
+}
+
 // Wave tile of L lanes per pixel (1, 2, 4, 8, 16): 64 / L pixels as
 // wave_bw x wave_bh, the L lanes of a pixel adjacent.
 SDTFE_HD constexpr int wave_bw(int L) { return L <= 2 ? 8 : L <= 8 ? 4 : 2; }
@@ -97,10 +104,14 @@ SDTFE_HD constexpr int wave_bh(int L) { return L == 1 ? 8 : L <= 4 ? 4 : 2; }
 // Lanes per pixel of a single map: the rays of a wave then start together
 // on a 2x2 (L = 16) or 4x4 (L = 4) pixel patch and share more records.
 // The mixed walk's hand-over window bounds the samples a split pixel may
-// have (kDeferWindow).
+// have: 64 with 16 lanes (one mask word), kDeferWindow with 4.
 SDTFE_HD constexpr int map_lanes(int samples) {
-  return samples > kDeferWindow ? 1 : samples >= 16 ? 16 : samples >= 4 ? 4 : 1;
+  return samples > kDeferWindow ? 1 : samples > 64 ? 4 : samples >= 16 ? 16 : samples >= 4 ? 4 : 1;
 }
+
+// Mask words of the hand-over window a map kernel with L lanes per pixel
+// keeps (map_lanes picks L accordingly).
+SDTFE_HD constexpr int map_window_words(int L) { return L == 4 ? 4 : 1; }
 
 // Pixel and sample slice j (0..L-1) of a lane with L lanes per pixel:
 // blocks of 256 lanes cover 2 x 2 wave tiles, remapped as in lane_pixel.

@@ -93,11 +93,25 @@ SDTFE_HD inline int deferred_sample(const Deferred *__restrict__ d, int k) {
   return d->tail + k;
 }
 
-// One sample of an entry through the FP64 walk: its value (0 for a miss or
-// a failed walk) and whether it crossed a tet.
-SDTFE_HD inline double deferred_walk(const Deferred *__restrict__ d, int s,
-                                     const Rec64 *__restrict__ rec64, const EntryGrid &eg,
-                                     const MapGeom &g, bool &ok, LaneCounters &cnt) {
+// What the second pass walks with: the tessellation's records, the face
+// table (LDS on the device) and the map.
+struct DeferCtx {
+  const Rec32 *rec32;
+  const Rec64 *rec64;
+  const FaceRec *frec;
+  const uint8_t *faces;
+  EntryGrid eg;
+  MapGeom g;
+  Filter flt;
+};
+
+// One sample of an entry: the mixed walk again (it hands the ray over where
+// it did in the first pass) and the FP64 walk from there; its value (0 for
+// a miss or a failed walk) and whether it crossed a tet.
+template <bool kWide>
+SDTFE_HD inline double deferred_walk(const Deferred *__restrict__ d, int s, const DeferCtx &c,
+                                     bool &ok, LaneCounters &cnt) {
+  const MapGeom &g = c.g;
   const uint64_t pix = d->pix;
   const int px = (int)(pix % (uint64_t)g.grid), py = (int)(pix / (uint64_t)g.grid);
   const uint64_t key = sample_key(g.seed, pix, (uint32_t)s);
@@ -105,14 +119,20 @@ SDTFE_HD inline double deferred_walk(const Deferred *__restrict__ d, int s,
   sample_point(g, px, py, key, xt, yt);
   if (s > d->tail) cnt.samples++;  // the first pass counted the others
   ok = false;
-  const int entry = locate_entry(eg, xt, yt);
+  const int entry = locate_entry(c.eg, xt, yt);
   if (entry < 0) {  // only tail samples can miss
     cnt.missed++;
     return 0.0;
   }
   int crossed;
-  const double v = walk64(rec64, entry, xt, yt, g, key, crossed, cnt);
-  cnt.fp64_redo++;
+  bool resumed;
+  const double v =
+      c.flt.full_slab
+          ? walk_mixed_resume<true, kWide>(c.rec32, c.rec64, c.frec, c.faces, entry, xt, yt, g,
+                                           c.flt, key, crossed, resumed, cnt)
+          : walk_mixed_resume<false, kWide>(c.rec32, c.rec64, c.frec, c.faces, entry, xt, yt, g,
+                                            c.flt, key, crossed, resumed, cnt);
+  if (resumed) cnt.fp64_redo++;
   cnt.tet_steps += (unsigned)crossed;
   if (crossed == 0) cnt.failed++;
   ok = crossed > 0;
@@ -123,18 +143,17 @@ SDTFE_HD inline double deferred_walk(const Deferred *__restrict__ d, int s,
 // Second pass for one entry by one wave: lane j walks the entry's samples
 // j, j + 64, ...; the values are added in sample order.  Every lane returns
 // the pixel's sample sum and n_ok (samples that crossed a tet).
-__device__ inline double finish_deferred_wave(const Deferred *__restrict__ d,
-                                              const Rec64 *__restrict__ rec64,
-                                              const EntryGrid &eg, const MapGeom &g, int &n_ok,
-                                              LaneCounters &cnt) {
+template <bool kWide>
+__device__ inline double finish_deferred_wave(const Deferred *__restrict__ d, const DeferCtx &c,
+                                              int &n_ok, LaneCounters &cnt) {
   const int lane = threadIdx.x & 63;
-  const int total = deferred_count(d, g.samples);
+  const int total = deferred_count(d, c.g.samples);
   double sum = d->sum;
   n_ok = d->n_ok;
   for (int r0 = 0; r0 < total; r0 += 64) {
     double v = 0.0;
     bool ok = false;
-    if (r0 + lane < total) v = deferred_walk(d, deferred_sample(d, r0 + lane), rec64, eg, g, ok, cnt);
+    if (r0 + lane < total) v = deferred_walk<kWide>(d, deferred_sample(d, r0 + lane), c, ok, cnt);
     const int m = min(64, total - r0);
     for (int i = 0; i < m; ++i) {
       sum += __shfl(v, i, 64);

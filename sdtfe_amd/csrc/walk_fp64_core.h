@@ -206,20 +206,27 @@ SDTFE_HD inline double tet_integral64(const Tet64 &r, double ox, double oy, doub
   return 0.0;
 }
 
-// shoot_3d_ray (src/dtfe.c:247-374) with perturb_ray (:217-245).  key names
-// the sample; perturbation corners are draws 2, 3, ... of it.
-SDTFE_HD inline double walk64(const Rec64 *__restrict__ rec, int entry, double x, double y,
-                         const MapGeom &g, uint64_t key, int &crossed, LaneCounters &cnt) {
+// shoot_3d_ray (src/dtfe.c:247-374) with perturb_ray (:217-245), from tet
+// start entered from tet from with steps0 tets crossed before (a walk from
+// the entry tet: start = from = entry, steps0 = 0, which matches the
+// reference's dummy first record).  key names the sample; perturbation
+// corners are draws 2, 3, ... of it.  crossed: the tets crossed by this
+// call (0 when the walk fails).
+SDTFE_HD inline double walk64_from(const Rec64 *__restrict__ rec, int start, int from,
+                                   int steps0, double x, double y, const MapGeom &g,
+                                   uint64_t key, int &crossed, LaneCounters &cnt) {
   crossed = 0;
-  if (entry < 0) return 0.0;
+  if (start < 0) return 0.0;
   const double max_dist = g.dxy / 2.0;
   const double step = max_dist / kPerturbMax;
   double ox = x, oy = y, acc = 0.0;
   uint32_t draw = 2;
-  int cur = entry, prev = -1, f_in = -1, f_out = 0, nxt = entry, steps = 0;
+  int cur = from, prev = -1, f_in = -1, f_out = 0, nxt = start, steps = steps0;
+  bool first = true;
   Tet64 r;
   for (;;) {
-    if (steps > 0) nxt = nb_of(r, f_out);
+    if (!first) nxt = nb_of(r, f_out);
+    first = false;
     if (nxt == -1) break;
     if (nxt == prev) {  // turn around (src/dtfe.c:303-309)
       const int tmp = f_in;
@@ -252,6 +259,12 @@ SDTFE_HD inline double walk64(const Rec64 *__restrict__ rec, int entry, double x
     acc += tet_integral64(r, ox, oy, h.z_in, h.z_out, g.off_z, g.lz);
   }
   return acc;
+}
+
+// The whole walk from the entry tet.
+SDTFE_HD inline double walk64(const Rec64 *__restrict__ rec, int entry, double x, double y,
+                              const MapGeom &g, uint64_t key, int &crossed, LaneCounters &cnt) {
+  return walk64_from(rec, entry, entry, 0, x, y, g, key, crossed, cnt);
 }
 
 }  // namespace sdtfe

@@ -19,7 +19,7 @@ __global__ void defer_reset_kernel(int *count) { *count = 0; }
 // start together at the top of one small patch of the map, so they share
 // more face records), and after each round the pixel's first lane takes the
 // L values in sample order, so the map does not depend on L.  With L > 1
-// the samples must fit the window of handed-over samples (kDeferWindow).
+// the samples must fit the window of handed-over samples (map_lanes).
 template <bool kFullSlab, bool kWide, int L>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDTFE_MIXED_WAVES))) void
 map_mixed_kernel(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec64,
@@ -33,7 +33,7 @@ map_mixed_kernel(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec6
   const bool active = px < g.grid && py < g.grid;
   const uint64_t pix = (uint64_t)py * (uint64_t)g.grid + (uint64_t)px;
   LaneCounters cnt;
-  PixelSamples<L == 1 ? 1 : 4> ps(g.samples);  // the pixel's first lane's
+  PixelSamples<map_window_words(L)> ps(g.samples);  // the pixel's first lane's
   // one sample: 1 walked (value v, crossed > 0), 0 adds nothing (a miss, a
   // failed walk), 2 handed over; -1 not walked
   auto sample = [&](int s, double &v) -> int {
@@ -48,8 +48,9 @@ map_mixed_kernel(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec6
       return 0;
     }
     int crossed;
+    Resume rs;  // the second pass walks the sample again and resumes there
     if (!walk_mixed<kFullSlab, kWide>(rec32, rec64, frec, faces, entry, xt, yt, g, mc.flt, v,
-                                      crossed, cnt))
+                                      crossed, cnt, rs))
       return 2;
     cnt.tet_steps += (unsigned)crossed;
     if (crossed == 0) cnt.failed++;
@@ -78,7 +79,7 @@ map_mixed_kernel(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec6
           if (ki == 1)
             ps.add(vi, 1);
           else if (ki == 2)
-            ps.hand_over(s0 + i);  // within the window: samples <= kDeferWindow
+            ps.hand_over(s0 + i);  // within the window (map_lanes)
         }
       }
     }
@@ -99,17 +100,23 @@ map_mixed_kernel(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec6
 }
 
 // Second pass: one wave per queued pixel, grid-stride over the queue.
-__global__ __launch_bounds__(256) void map_deferred_kernel(const Rec64 *__restrict__ rec64,
-                                                          EntryGrid eg, MapGeom g, double *out,
-                                                          int *navg, int accumulate,
+template <bool kWide>
+__global__ __launch_bounds__(256) void map_deferred_kernel(const Rec32 *__restrict__ rec32,
+                                                          const Rec64 *__restrict__ rec64,
+                                                          const FaceRec *__restrict__ frec,
+                                                          EntryGrid eg, MapGeom g, MixedConsts mc,
+                                                          double *out, int *navg, int accumulate,
                                                           DeferQueue dq,
                                                           unsigned long long *stats) {
+  __shared__ uint8_t faces[kFaceCodes];
+  load_faces(faces);
+  const DeferCtx c{rec32, rec64, frec, faces, eg, g, mc.flt};
   LaneCounters cnt;
   const int n = min(*dq.count, dq.cap);
   const int waves = gridDim.x * (blockDim.x >> 6);
   for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
     int n_ok;
-    const double sum = finish_deferred_wave(dq.q + i, rec64, eg, g, n_ok, cnt);
+    const double sum = finish_deferred_wave<kWide>(dq.q + i, c, n_ok, cnt);
     if ((threadIdx.x & 63) == 0) {
       const uint64_t pix = dq.q[i].pix;
       const double val = n_ok > 0 ? sum / n_ok : 0.0;
@@ -123,8 +130,8 @@ __global__ __launch_bounds__(256) void map_deferred_kernel(const Rec64 *__restri
   cnt.flush(stats);
 }
 
-// Rays one by one; a handed-over ray is walked in FP64 in place (this
-// kernel serves inspection, not maps).
+// Rays one by one; a handed-over ray goes on in the FP64 walk in place
+// (this kernel serves inspection, not maps).
 __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
                                    const Rec64 *__restrict__ rec64,
                                    const FaceRec *__restrict__ frec, EntryGrid eg, MapGeom g,
@@ -140,16 +147,14 @@ __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
     const uint64_t key = sample_key(g.seed, pixel[i], sample[i]);
     const int entry = entry_in ? entry_in[i] : locate_entry(eg, x[i], y[i]);
     int crossed;
-    double v;
-    const bool done = mc.flt.full_slab
-                          ? walk_mixed<true, true>(rec32, rec64, frec, faces, entry, x[i], y[i], g,
-                                                   mc.flt, v, crossed, cnt)
-                          : walk_mixed<false, true>(rec32, rec64, frec, faces, entry, x[i], y[i], g,
-                                                    mc.flt, v, crossed, cnt);
-    if (!done) {
-      cnt.fp64_redo++;
-      v = walk64(rec64, entry, x[i], y[i], g, key, crossed, cnt);
-    }
+    bool resumed;
+    const double v =
+        mc.flt.full_slab
+            ? walk_mixed_resume<true, true>(rec32, rec64, frec, faces, entry, x[i], y[i], g, mc.flt,
+                                            key, crossed, resumed, cnt)
+            : walk_mixed_resume<false, true>(rec32, rec64, frec, faces, entry, x[i], y[i], g,
+                                             mc.flt, key, crossed, resumed, cnt);
+    if (resumed) cnt.fp64_redo++;
     value[i] = v;
     crossed_out[i] = crossed;
     cnt.tet_steps += (unsigned)crossed;
@@ -221,8 +226,12 @@ int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
     SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
     launch_first_pass(t, g, mc, out, navg, accumulate, dq);
     SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
-    map_deferred_kernel<<<1024, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg,
-                                                     accumulate, dq, t->stats);
+    if ((int64_t)t->nt > kFaceRecNarrowTets)
+      map_deferred_kernel<true><<<1024, 256, 0, t->stream>>>(
+          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+    else
+      map_deferred_kernel<false><<<1024, 256, 0, t->stream>>>(
+          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
     SDTFE_HIP(hipGetLastError());
     int count = 0;
     SDTFE_HIP(hipMemcpyAsync(&count, t->defer_count, sizeof(int), hipMemcpyDeviceToHost,

@@ -1,18 +1,18 @@
 // walk_mixed_core.h -- the mixed-precision ray walk (gfx950).
 //
 // Same tets, same faces, same perturbations as the FP64 walk, with far fewer
-// loads and instructions per tet.  On MI355X the walk is limited by the
-// vector memory pipeline (every lane of a wave gathers its own record) and
-// by instruction issue (an unpacked FP32 operation costs what an FP64 one
-// does), so the design cuts both:
+// loads and instructions per tet.  On MI355X the walk is limited by
+// instruction issue (an FP64 or conversion operation takes twice the issue
+// time of an FP32 one) and by the vector memory pipeline (every lane of a
+// wave gathers its own record), so the design cuts both:
 //
 //  * Carried face.  The ray enters each tet through a face whose three
 //    vertices it already knows: their offsets from the ray (x, y in FP32,
-//    formed from FP64 when the vertex first appears, so the rounding is a
-//    fraction of the local tet size), heights (FP64) and densities, and the
-//    face's three edge sides.  A step loads one 32-byte face record
-//    (sdtfe::FaceRec: the next tet and its fourth vertex) and forms only the
-//    three sides of the new vertex.
+//    rounded relative to the offset itself, so the error is a fraction of
+//    the local tet size), heights above the carried crossing (FP32, rebased
+//    every step) and densities, and the face's three edge sides.  A step
+//    loads one 32-byte face record (sdtfe::FaceRec: the next tet and its
+//    fourth vertex) and forms only the three sides of the new vertex.
 //  * Side signs in FP32 with a filter.  The Plücker product of the ray with
 //    edge (X, Y) is lz * orient2d(p; X, Y) (src/dtfe.c:24-26 expanded),
 //    formed as (X - p) x (Y - p).  With M the largest offset coordinate of
@@ -30,13 +30,15 @@
 //    go through the same filter and a 64-entry sign-mask table (face_code).
 //  * Crossings.  The exit crossing (height and density, barycentric in the
 //    face's three vertices with the weights the FP64 walk uses,
-//    src/dtfe.c:67-74) is formed in FP32 relative to a face vertex and
+//    src/dtfe.c:67-74) is formed in FP32 relative to the entry crossing and
 //    carried as the next tet's entry, so errors cancel to first order (the
 //    DTFE field is continuous across the face); the field is linear along the
 //    chord, so the trapezoid rule on the two crossings is its exact integral.
 //    The ray's first and last crossings (no neighbour to cancel against; hull
 //    tets are large), crossings of nearly vertical faces and midpoints at the
 //    slab boundary are formed in FP64.
+//  * One exit per step, at its top, so the loop's control flow stays simple
+//    (a hand-over decided inside a step is carried to the next one's test).
 //  Per-ray and per-pixel sums stay FP64; volumes and vertex densities are the
 //  FP64 ones.
 #pragma once
@@ -89,6 +91,11 @@ struct Filter {
   int full_slab;  // the slab holds the hull's z range: every midpoint is inside
 };
 
+// The filter's bound on a side's FP32 error, plus the zero band.
+SDTFE_HD inline float side_bound(const Filter &flt, float M) {
+  return fmaf(flt.k_err, M * M, flt.thr);
+}
+
 // Reciprocal to within 1 ulp (v_rcp_f32) on the device.
 SDTFE_HD inline float rcp32(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -123,8 +130,10 @@ SDTFE_HD inline FaceRec load_facerec(const FaceRec *__restrict__ frec, int t, in
             : reinterpret_cast<const uint4 *>(base + (((uint32_t)t << 7) | (uint32_t)(f8 << 2)));
   const uint4 a = p[0], b = p[1];
   FaceRec r;
-  r.qx = __builtin_bit_cast(double, ((uint64_t)a.y << 32) | a.x);
-  r.qy = __builtin_bit_cast(double, ((uint64_t)a.w << 32) | a.z);
+  r.qxh = __builtin_bit_cast(float, a.x);
+  r.qxl = __builtin_bit_cast(float, a.y);
+  r.qyh = __builtin_bit_cast(float, a.z);
+  r.qyl = __builtin_bit_cast(float, a.w);
   r.nb = (int)b.x;
   r.code = b.y;
   r.dqz = __builtin_bit_cast(float, b.z);
@@ -151,6 +160,7 @@ struct Cross64 {
   double z;
   float rho;
   double cond;  // sum of the weights' magnitudes over their sum's
+  double zsum;  // the three vertices' heights, (z0 + z1) + z2
 };
 
 SDTFE_HD inline Cross64 face_cross64(const Rec64 *__restrict__ rec, int t, int f, double x,
@@ -171,6 +181,7 @@ SDTFE_HD inline Cross64 face_cross64(const Rec64 *__restrict__ rec, int t, int f
   c.z = z0 + w1 * (z1 - z0) + w2 * (z2 - z0);
   c.rho = (float)(r0 + w1 * (r1 - r0) + w2 * (r2 - r0));
   c.cond = (fabs(s0) + fabs(s1) + fabs(s2)) / fabs(sum);
+  c.zsum = (z0 + z1) + z2;
   return c;
 }
 
@@ -219,6 +230,14 @@ struct Slot {
   int lab8;
 };
 
+// The carried crossing's height from the FP64 heights of the carried face's
+// vertices and their FP32 heights above it (the full-slab walk does not
+// keep it: nothing else needs it).
+SDTFE_HD inline double carried_z(const Cross64 &c, const Slot &p0, const Slot &p1,
+                                 const Slot &p2) {
+  return (c.zsum - (double)((p0.z + p1.z) + p2.z)) * (1.0 / 3.0);
+}
+
 // m ? a : b per lane, without control flow (see pick4).
 SDTFE_HD inline float self(bool m, float a, float b) {
   return __builtin_bit_cast(float, bsel32(0u - (uint32_t)m, __builtin_bit_cast(uint32_t, a),
@@ -234,10 +253,20 @@ SDTFE_HD inline Slot sel_slot(bool m, const Slot &a, const Slot &b) {
               seli(m, a.lab8, b.lab8)};
 }
 
+// Where the FP64 walk takes over a ray the mixed walk hands over: tet t,
+// entered through its face enter after steps tets, with acc (twice the
+// integral so far, as the walk keeps it); enter < 0: the whole ray.  The
+// mixed walk crossed the FP64 walk's tets up to t and no perturbation
+// happened before it, so walk64_from(t) continues the FP64 walk exactly.
+struct Resume {
+  int t, enter, steps;
+  double acc;
+};
+
 // One sample-ray.  (x, y) FP64 sample position.  Returns false when the ray
-// must be walked by the FP64 walk instead (walk64 with the same key): a zero
-// side (a perturbation), a failed face choice, an inconsistent face record
-// or a midpoint on the slab boundary.  Otherwise value and crossed are the
+// must be walked by the FP64 walk instead, from where rs says (walk64_from
+// with the same key): a zero side (a perturbation), a failed face choice, an
+// inconsistent face record or a midpoint on the slab boundary.  Otherwise value and crossed are the
 // ray's integral and tet count, and the tets are the FP64 walk's.  faces:
 // the kFaceCodes face_code values (LDS on the device).  kFullSlab: the slab
 // holds the hull's z range (Filter::full_slab), no midpoint test.  kWide:
@@ -246,7 +275,7 @@ template <bool kFullSlab, bool kWide>
 SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec64,
                                 const FaceRec *__restrict__ frec, const uint8_t *faces, int entry,
                                 double x, double y, const MapGeom &g, const Filter &flt,
-                                double &value, int &crossed, LaneCounters &cnt) {
+                                double &value, int &crossed, LaneCounters &cnt, Resume &rs) {
   value = 0.0;
   crossed = 0;
   if (entry < 0) return true;
@@ -260,7 +289,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
   Slot p0, p1, p2;
   float e0, e1, e2;  // sides of the carried face: (P0, P1), (P1, P2), (P2, P0)
   int enter, f8;     // t's entry face, 8 x its exit face
-  double zc;         // carried crossing: absolute height
+  double zc;         // carried crossing: absolute height (kept by the slab walk)
   float rc;          // and density
   float zext;        // t's height range
   {
@@ -271,7 +300,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     const float cx = (float)(q.cx - x), cy = (float)(q.cy - y);
     const float dx = (float)(q.dx - x), dy = (float)(q.dy - y);
     const float M = amax(amax(absmax3(ax, ay, bx), absmax3(by, cx, cy)), absmax3(dx, dy, 0.f));
-    const float bound = fmaf(flt.k_err, M * M, flt.thr);
+    const float bound = side_bound(flt, M);
     const float cs[6] = {side32(cx, cy, bx, by), side32(cx, cy, ax, ay), side32(bx, by, ax, ay),
                          side32(cx, cy, dx, dy), side32(bx, by, dx, dy), side32(dx, dy, ax, ay)};
     constexpr int kX[6] = {2, 2, 1, 2, 1, 3}, kY[6] = {1, 0, 0, 3, 3, 0};  // CB AC BA DC BD DA
@@ -285,7 +314,10 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
       w |= (uint32_t)(sg < 0) << e;
     }
     const int code = faces[w];
-    if (zero || !(code & 16)) return false;
+    if (zero || !(code & 16)) {
+      rs = Resume{entry, -1, 0, 0.0};
+      return false;
+    }
     enter = code & 3;
     const int f = (code >> 2) & 3;
     f8 = 8 * f;
@@ -324,8 +356,12 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
   // the face update and the exit crossing.  Each step has one exit, at its
   // top: the hull, a record without back link, the step bound, or a
   // hand-over decided in the step before (steps = -1).
+  float xh, xl, yh, yl;  // the ray as FP32 pairs (split32)
+  split32(x, xh, xl);
+  split32(y, yh, yl);
   double acc = 0.0;  // sum of (rho_in + rho_out) dz: twice the integral
-  int steps = 1;     // tets entered
+  int steps = 1;     // tets entered; ~(tets before t) once handed over
+  bool slab_fail = false;
   FaceRec R = load_facerec<kWide>(frec, t, f8);
   for (;;) {
     // exit crossing in FP32, height above the entry crossing (does not wait
@@ -339,7 +375,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     if (!stop && fabsf(den) < flt.k_ill * M * M) {
       // rare: FP32 weights of a nearly vertical face are too uncertain
       const Cross64 o = face_cross64(rec64, t, f8 >> 3, x, y, load_rho4(rec32, t));
-      dz = (float)(o.z - zc);
+      dz = (float)(o.z - (kFullSlab ? carried_z(o, p0, p1, p2) : zc));
       r_out = o.rho;
     }
     if (stop) break;
@@ -352,7 +388,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
                        inside);
       if (inside) acc += (double)((rc + r_out) * dz);
     }
-    zc += (double)dz;
+    if (!kFullSlab) zc += (double)dz;
     rc = r_out;
     p0.z -= dz;
     p1.z -= dz;
@@ -360,8 +396,10 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
 
     // enter the next tet through its face `back`; its fourth vertex Q
     const int back8 = (int)(R.code & 0x18u);
-    Slot qv{(float)(R.qx - x), (float)(R.qy - y), fmaf((p0.z + p1.z) + p2.z, 1.0f / 3.0f, R.dqz),
-            R.rho, back8};
+    // offsets from the FP32 pairs: hi - hi rounds once relative to the
+    // offset itself, lo - lo is far below it
+    Slot qv{(R.qxh - xh) + (R.qxl - xl), (R.qyh - yh) + (R.qyl - yl),
+            fmaf((p0.z + p1.z) + p2.z, 1.0f / 3.0f, R.dqz), R.rho, back8};
     // labels of the carried vertices in the next tet
     p0.lab8 = (int)((R.code >> (p0.lab8 ^ f8)) & 31u);
     p1.lab8 = (int)((R.code >> (p1.lab8 ^ f8)) & 31u);
@@ -374,7 +412,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     float s1 = side32(p1.ox, p1.oy, qv.ox, qv.oy);
     float s2 = side32(p2.ox, p2.oy, qv.ox, qv.oy);
     const float Mt = amax(amax(M, fabsf(qv.ox)), fabsf(qv.oy));
-    const float bound = fmaf(flt.k_err, Mt * Mt, flt.thr);
+    const float bound = side_bound(flt, Mt);
     bool zero = false;
     if (!(fminf(fminf(fabsf(s0), fabsf(s1)), fabsf(s2)) > bound)) {
       // rare: the FP64 walk's signs for the sides the filter leaves open
@@ -412,20 +450,30 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     p1 = sel_slot(r1, qv, p1);
     p2 = sel_slot(r2, qv, p2);
     // with exact signs exactly one of r0, r1, r2 holds
+    if (!kFullSlab) slab_fail = slab_fail || !ok;
     const bool fail = zero || !(r0 || r1 || r2) || !ok;
-    steps = fail ? -1 : steps + 1;
+    steps = fail ? ~steps : steps + 1;
   }
-  if (steps < 0) return false;
+  if (steps < 0) {
+    // t is the tet just entered (a midpoint left open: the whole ray)
+    rs = (!kFullSlab && slab_fail) ? Resume{entry, -1, 0, 0.0} : Resume{t, enter, ~steps, acc};
+    return false;
+  }
   if ((int)R.code < 0) {
-    if (R.nb >= 0) return false;  // the mesh lacks the back link
+    if (R.nb >= 0) {  // the mesh lacks the back link: the FP64 walk from t
+      rs = Resume{t, enter, steps - 1, acc};
+      return false;
+    }
     // the last crossing in FP64, where the lanes of a wave are together again
     const Cross64 out = face_cross64(rec64, t, f8 >> 3, x, y, load_rho4(rec32, t));
-    const double dz = out.z - zc;
+    const double dz = out.z - (kFullSlab ? carried_z(out, p0, p1, p2) : zc);
     bool inside = true;
     if (!kFullSlab) {
       const float ze = amax(amax(p0.z, p1.z), p2.z) - fminf(fminf(p0.z, p1.z), p2.z);
-      if (!slab_inside(rec64, rec32, t, enter, f8 >> 3, x, y, ze, zc, out.z, g, flt, inside))
+      if (!slab_inside(rec64, rec32, t, enter, f8 >> 3, x, y, ze, zc, out.z, g, flt, inside)) {
+        rs = Resume{t, enter, steps - 1, acc};
         return false;
+      }
     }
     if (inside) acc += (double)(rc + out.rho) * dz;
     value = 0.5 * acc;
@@ -434,6 +482,32 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
   }
   cnt.step_cap++;  // the step bound
   return true;
+}
+
+// The mixed walk, and where it hands the ray over, the FP64 walk from
+// there: the value and tets crossed (0 for a failed walk).  resumed: the
+// FP64 walk took part.
+template <bool kFullSlab, bool kWide>
+SDTFE_HD inline double walk_mixed_resume(const Rec32 *__restrict__ rec32,
+                                         const Rec64 *__restrict__ rec64,
+                                         const FaceRec *__restrict__ frec, const uint8_t *faces,
+                                         int entry, double x, double y, const MapGeom &g,
+                                         const Filter &flt, uint64_t key, int &crossed,
+                                         bool &resumed, LaneCounters &cnt) {
+  double v;
+  Resume rs;
+  resumed = false;
+  if (walk_mixed<kFullSlab, kWide>(rec32, rec64, frec, faces, entry, x, y, g, flt, v, crossed,
+                                   cnt, rs))
+    return v;
+  resumed = true;
+  if (rs.enter < 0) return walk64(rec64, entry, x, y, g, key, crossed, cnt);
+  Tet64 q;
+  load_tet(rec64, rs.t, q);
+  int c;
+  const double w = walk64_from(rec64, rs.t, nb_of(q, rs.enter), rs.steps, x, y, g, key, c, cnt);
+  crossed = c > 0 ? rs.steps + c : 0;
+  return c > 0 ? 0.5 * rs.acc + w : 0.0;
 }
 
 struct MixedConsts {
@@ -447,10 +521,13 @@ inline MixedConsts mixed_consts(const sdtfe_tess *t, const MapGeom &g) {
   const double amax = std::max(t->span + std::max(std::fabs(t->org[0]), std::fabs(t->org[1])),
                                g.ray_span + std::max(std::fabs(g.org_x), std::fabs(g.org_y)));
   MixedConsts m;
-  // Offsets X - p round twice (u32 + 2 u64): every component is within
-  // 1.0001 u32 M.  A side's fma and product add 3 u32 M^2, the offset errors
-  // 4 (u32 M) M; 8 pads the bound's own rounding.
-  m.flt.k_err = (float)(8.0 * u32);
+  // An offset X - p is within 2.0001 u32 M of exact: from FP64 it rounds
+  // once (plus u64 terms); from the face records' FP32 pairs (split32) the
+  // hi difference and the final sum round once each relative to the offset,
+  // the lo terms and the pairs' own error are below 2^-46 of the
+  // coordinates.  A side's fma and product add 3 u32 M^2, the offset errors
+  // 4 (2.0001 u32 M) M; 12 pads the bound's own rounding.
+  m.flt.k_err = (float)(12.0 * u32);
   // reference zero band and the FP64 expression's own rounding (three
   // products and two sums of terms below 2 lz amax^2), over lz
   const double err64 = 16.0 * std::ldexp(1.0, -53) * 2.0 * g.lz * amax * amax;

@@ -93,23 +93,22 @@ int main(int argc, char **argv) {
     const uint64_t key = sdtfe::sample_key(seed, pix[i], (uint32_t)smp[i]);
     double v = 0.0;
     // the mixed walk hands some rays to the FP64 walk, as the map kernels do
-    bool done = false;
     if (mixed) {
+      // the mixed walk, and the FP64 walk from where it hands a ray over,
+      // as the map kernels' second pass does
       const auto *r32 = rec32.data();
       const auto *r64 = rec.data();
       const auto *fr = frec.data();
       const int e = (int)ent[i];
+      bool resumed = false;
       if (wide)
-        done = mc.flt.full_slab
-                   ? sdtfe::walk_mixed<true, true>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, v, crossed, c)
-                   : sdtfe::walk_mixed<false, true>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, v, crossed, c);
+        v = mc.flt.full_slab ? sdtfe::walk_mixed_resume<true, true>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, key, crossed, resumed, c)
+                             : sdtfe::walk_mixed_resume<false, true>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, key, crossed, resumed, c);
       else
-        done = mc.flt.full_slab
-                   ? sdtfe::walk_mixed<true, false>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, v, crossed, c)
-                   : sdtfe::walk_mixed<false, false>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, v, crossed, c);
-    }
-    if (!done) {
-      if (mixed) c.fp64_redo++;
+        v = mc.flt.full_slab ? sdtfe::walk_mixed_resume<true, false>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, key, crossed, resumed, c)
+                             : sdtfe::walk_mixed_resume<false, false>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, key, crossed, resumed, c);
+      if (resumed) c.fp64_redo++;
+    } else {
       v = sdtfe::walk64(rec.data(), (int)ent[i], x[i], y[i], g, key, crossed, c);
     }
     redo += c.fp64_redo;

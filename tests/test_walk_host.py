@@ -53,14 +53,15 @@ def records32(pos, tetra, rho):
 
 def face_records(pos, tetra, rho):
     """The mixed walk's 32-byte face records (4 per tet): the new vertex
-    (x, y FP64), the tet across face f, a code (8 x the face's index there in
-    bits 3-4, 8 x the labels there of this tet's vertices f^k in bits
-    8k..8k+4, bit 31 when the walk stops: hull or missing back link), and
-    the new vertex's z relative to the mean z of face f's vertices and its
-    density (FP32)."""
+    (x, y as FP32 hi/lo pairs), the tet across face f, a code (8 x the face's
+    index there in bits 3-4, 8 x the labels there of this tet's vertices f^k
+    in bits 8k..8k+4, bit 31 when the walk stops: hull or missing back
+    link), and the new vertex's z relative to the mean z of face f's
+    vertices and its density (FP32)."""
     t = tetra.shape[0]
-    rec = np.zeros((t, 4), dtype=np.dtype([("qx", "<f8"), ("qy", "<f8"), ("nb", "<i4"),
-                                           ("code", "<u4"), ("dqz", "<f4"), ("rho", "<f4")]))
+    rec = np.zeros((t, 4), dtype=np.dtype([("qxh", "<f4"), ("qxl", "<f4"), ("qyh", "<f4"),
+                                           ("qyl", "<f4"), ("nb", "<i4"), ("code", "<u4"),
+                                           ("dqz", "<f4"), ("rho", "<f4")]))
     assert rec.dtype.itemsize == 32
     stop = np.int64(1) << 31
     verts, nbrs = tetra[:, :4], tetra[:, 4:]
@@ -84,8 +85,11 @@ def face_records(pos, tetra, rho):
             code |= (lab & 3) << (8 * k + 3)
         q = verts[nn, back & 3]
         rec["code"][:, f] = np.where(ok, code, stop)
-        rec["qx"][:, f] = np.where(ok, pos[q, 0], 0)
-        rec["qy"][:, f] = np.where(ok, pos[q, 1], 0)
+        for c, k in ((0, "qx"), (1, "qy")):
+            hi = pos[q, c].astype(np.float32)
+            lo = (pos[q, c] - hi.astype(np.float64)).astype(np.float32)
+            rec[k + "h"][:, f] = np.where(ok, hi, 0)
+            rec[k + "l"][:, f] = np.where(ok, lo, 0)
         rec["dqz"][:, f] = np.where(ok, (pos[q, 2] - zs / 3.0).astype(np.float32), 0)
         rec["rho"][:, f] = np.where(ok, rho[q].astype(np.float32), 0)
     return rec.reshape(-1)
