@@ -368,8 +368,9 @@ def main():
         # past the HBM peak: the kernel is not HBM-bound
         roofline["hbm_bound"] = False
         roofline["note"] = ("algorithmic bytes/s exceed the HBM peak because the rays of a wave "
-                            "re-read the same tets from L2/MALL; physical HBM traffic is "
-                            "physical_gbs; the walk is bound by instruction issue")
+                            "re-read the same face records from the vector L1 and L2; physical "
+                            "HBM traffic is physical_gbs; the walk is bound by the vector "
+                            "memory pipeline's data return (two record loads per tet-step)")
 
     cpu = None
     if not a.no_cpu:

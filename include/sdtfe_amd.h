@@ -179,7 +179,7 @@ int sdtfe_batch_run(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
 /* A map server with continuous batching.  Requests (particles, tets, map
  * parameters) are copied at submit and admitted in arrival order while their
  * estimated HBM footprint fits hbm_budget_bytes and fewer than max_active are
- * resident; a setup thread builds their tessellations while a walk thread
+ * resident; setup threads build their tessellations while a walk thread
  * runs persistent launches over the next tiles (at most tile_budget per
  * launch, shared among resident requests), so requests arriving while others
  * are on the GPU join the next launch.  Each map is identical alone and in a
@@ -194,6 +194,7 @@ typedef struct {
   int lanes_per_pixel; /* 1, 2, 4, 8, 16; <= 0: by samples per pixel S (16 for
                           16 <= S <= 256, 4 for 4 <= S < 16, else 1) */
   int precision;       /* SDTFE_PREC_*, for every request */
+  int setup_threads;   /* requests set up concurrently; <= 0: 4 */
 } sdtfe_server_config;
 typedef struct {
   int status;

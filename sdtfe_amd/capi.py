@@ -48,7 +48,8 @@ class TessInfo(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int),
-                ("tile_budget", C.c_int), ("lanes_per_pixel", C.c_int), ("precision", C.c_int)]
+                ("tile_budget", C.c_int), ("lanes_per_pixel", C.c_int), ("precision", C.c_int),
+                ("setup_threads", C.c_int)]
 
 
 class RequestInfo(C.Structure):
@@ -346,9 +347,9 @@ class Server:
     """Map server with continuous batching (sdtfe_server)."""
 
     def __init__(self, hbm_budget_bytes=64 << 30, max_active=64, tile_budget=4096,
-                 lanes_per_pixel=0, precision=PREC_FP64):
+                 lanes_per_pixel=0, precision=PREC_FP64, setup_threads=0):
         cfg = ServerConfig(int(hbm_budget_bytes), int(max_active), int(tile_budget),
-                           int(lanes_per_pixel), int(precision))
+                           int(lanes_per_pixel), int(precision), int(setup_threads))
         h = C.c_void_p()
         check(lib().sdtfe_server_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -103,13 +104,13 @@ int check_params(const sdtfe_map_params *p) {
 
 int ensure_map_buf(sdtfe_tess *t, size_t npix) {
   if (npix <= t->map_cap) return 0;
-  if (t->map_buf) (void)hipFree(t->map_buf);
-  if (t->navg_buf) (void)hipFree(t->navg_buf);
+  dev_free(t->map_buf, t->stream);
+  dev_free(t->navg_buf, t->stream);
   t->map_buf = nullptr;
   t->navg_buf = nullptr;
   t->map_cap = 0;
-  if (hipMalloc((void **)&t->map_buf, sizeof(double) * npix) != hipSuccess ||
-      hipMalloc((void **)&t->navg_buf, sizeof(int) * npix) != hipSuccess)
+  if (dev_malloc((void **)&t->map_buf, sizeof(double) * npix, t->stream) != hipSuccess ||
+      dev_malloc((void **)&t->navg_buf, sizeof(int) * npix, t->stream) != hipSuccess)
     return fail(SDTFE_ENOMEM, "map buffer allocation failed");
   t->map_cap = npix;
   return 0;
@@ -137,15 +138,88 @@ struct DevArray {
   }
 };
 
+std::once_flag g_pool_kept[64];
+
+int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool given_field,
+           Staging *st, sdtfe_tess **out);
+
+}  // namespace
+
+hipError_t dev_malloc(void **p, size_t bytes, hipStream_t s) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::call_once(g_pool_kept[dev & 63], [dev] {
+    // keep freed blocks in the pool: later tessellations reuse them
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+  });
+  return hipMallocAsync(p, bytes ? bytes : 1, s);
+}
+
+void dev_free(void *p, hipStream_t s) {
+  if (p) (void)hipFreeAsync(p, s);
+}
+
+int staging_init(Staging &st, size_t bytes) {
+  st.bytes = bytes;
+  for (int k = 0; k < 2; ++k) {
+    if (hipHostMalloc(&st.buf[k], bytes, hipHostMallocDefault) != hipSuccess)
+      return fail(SDTFE_ENOMEM, "pinned staging allocation failed");
+    if (hipEventCreateWithFlags(&st.ev[k], hipEventDisableTiming) != hipSuccess)
+      return fail(SDTFE_EHIP, "staging event");
+  }
+  return 0;
+}
+
+void staging_free(Staging &st) {
+  for (int k = 0; k < 2; ++k) {
+    if (st.ev[k]) {
+      (void)hipEventSynchronize(st.ev[k]);
+      (void)hipEventDestroy(st.ev[k]);
+    }
+    if (st.buf[k]) (void)hipHostFree(st.buf[k]);
+    st.ev[k] = nullptr;
+    st.buf[k] = nullptr;
+  }
+}
+
+int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st) {
+  if (!st || !st->bytes) {
+    SDTFE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return 0;
+  }
+  for (size_t off = 0, k = 0; off < bytes; off += st->bytes, k ^= 1) {
+    const size_t n = std::min(st->bytes, bytes - off);
+    SDTFE_HIP(hipEventSynchronize(st->ev[k]));  // the buffer's previous copy is done
+    std::memcpy(st->buf[k], static_cast<const char *>(src) + off, n);
+    SDTFE_HIP(hipMemcpyAsync(static_cast<char *>(dst) + off, st->buf[k], n,
+                             hipMemcpyHostToDevice, s));
+    SDTFE_HIP(hipEventRecord(st->ev[k], s));
+  }
+  return 0;
+}
+
+int tess_create_staged(const double *particle_data, int n, const int *tetra_data, int nt,
+                       float p_mass, Staging *st, sdtfe_tess **out) {
+  return create(particle_data, n, tetra_data, nt, p_mass, false, st, out);
+}
+
+namespace {
+
 void free_tess(sdtfe_tess *t) {
   if (!t) return;
   void *bufs[] = {t->pos4,    t->tets,     t->vol,     t->nabla,   t->rec64,
                   t->rec32,   t->facerec,   t->tri_xy,   t->tri_tet, t->tri_abc, t->cell_off,
                   t->cell_tri, t->stats,   t->map_buf, t->navg_buf, t->defer_buf,
                   t->defer_count};
-  if (t->stream) (void)hipStreamSynchronize(t->stream);
-  for (void *b : bufs)
-    if (b) (void)hipFree(b);
+  if (t->stream) {
+    (void)hipStreamSynchronize(t->stream);
+    for (void *b : bufs) dev_free(b, t->stream);
+    (void)hipStreamSynchronize(t->stream);
+  }
   if (t->ev0) (void)hipEventDestroy(t->ev0);
   if (t->ev1) (void)hipEventDestroy(t->ev1);
   if (t->evk0) (void)hipEventDestroy(t->evk0);
@@ -155,7 +229,7 @@ void free_tess(sdtfe_tess *t) {
 }
 
 int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool given_field,
-           sdtfe_tess **out) {
+           Staging *st, sdtfe_tess **out) {
   if (!out) return fail(SDTFE_EINVAL, "null output handle");
   *out = nullptr;
   if (n < 0 || nt < 0) return fail(SDTFE_EINVAL, "negative size");
@@ -174,9 +248,10 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
   if (!rc) rc = hip_check(hipEventCreate(&t->ev1), "event");
   if (!rc) rc = hip_check(hipEventCreate(&t->evk0), "event");
   if (!rc) rc = hip_check(hipEventCreate(&t->evk1), "event");
-  if (!rc && hipMalloc((void **)&t->stats, sizeof(unsigned long long) * ST_COUNT) != hipSuccess)
+  if (!rc && dev_malloc((void **)&t->stats, sizeof(unsigned long long) * ST_COUNT, t->stream) !=
+                 hipSuccess)
     rc = fail(SDTFE_ENOMEM, "stats allocation failed");
-  if (!rc) rc = build_field(t, pd, tetra);
+  if (!rc) rc = build_field(t, pd, tetra, st);
   if (!rc) rc = build_entry_grid(t);
   if (rc) {
     free_tess(t);
@@ -224,12 +299,12 @@ uint64_t sdtfe_get_seed(void) { return g_seed.load(); }
 
 int sdtfe_tess_create(const double *particle_data, int n_particles, const int *tetra_data,
                       int n_tetra, float p_mass, sdtfe_tess **out) {
-  return create(particle_data, n_particles, tetra_data, n_tetra, p_mass, false, out);
+  return create(particle_data, n_particles, tetra_data, n_tetra, p_mass, false, nullptr, out);
 }
 
 int sdtfe_tess_create_field(const double *particle_data, int n_particles, const int *tetra_data,
                             int n_tetra, sdtfe_tess **out) {
-  return create(particle_data, n_particles, tetra_data, n_tetra, 1.0f, true, out);
+  return create(particle_data, n_particles, tetra_data, n_tetra, 1.0f, true, nullptr, out);
 }
 
 void sdtfe_tess_destroy(sdtfe_tess *t) { free_tess(t); }

@@ -396,23 +396,27 @@ namespace {
 template <typename T>
 int dev_alloc(sdtfe_tess *t, T **p, size_t count) {
   size_t bytes = sizeof(T) * (count > 0 ? count : 1);
-  hipError_t e = hipMalloc((void **)p, bytes);
-  if (e != hipSuccess) return fail(SDTFE_ENOMEM, "hipMalloc failed (" + std::to_string(bytes) +
+  hipError_t e = dev_malloc((void **)p, bytes, t->stream);
+  if (e != hipSuccess) return fail(SDTFE_ENOMEM, "device allocation failed (" +
+                                                     std::to_string(bytes) +
                                                      " B): " + hipGetErrorString(e));
   t->device_bytes += (int64_t)bytes;
   return 0;
 }
 
+// Temporaries of one setup, freed in stream order when it ends.
 struct Scratch {
+  hipStream_t s;
   std::vector<void *> ptrs;
+  explicit Scratch(hipStream_t stream) : s(stream) {}
   ~Scratch() {
-    for (void *p : ptrs) (void)hipFree(p);
+    for (void *p : ptrs) dev_free(p, s);
   }
   template <typename T>
   int get(T **p, size_t count) {
     size_t bytes = sizeof(T) * (count > 0 ? count : 1);
-    hipError_t e = hipMalloc((void **)p, bytes);
-    if (e != hipSuccess) return fail(SDTFE_ENOMEM, "hipMalloc (scratch) failed");
+    hipError_t e = dev_malloc((void **)p, bytes, s);
+    if (e != hipSuccess) return fail(SDTFE_ENOMEM, "device allocation (scratch) failed");
     ptrs.push_back(*p);
     return 0;
   }
@@ -430,15 +434,15 @@ inline unsigned blocks_for(int64_t n, int threads) {
     if (_rc) return _rc;   \
   } while (0)
 
-int build_field(sdtfe_tess *t, const double *pd, const int *tetra) {
+int build_field(sdtfe_tess *t, const double *pd, const int *tetra, Staging *st) {
   const int n = t->n, nt = t->nt;
   hipStream_t s = t->stream;
   SDTFE_RC(dev_alloc(t, &t->pos4, 4 * (size_t)n));
   SDTFE_RC(dev_alloc(t, &t->tets, 8 * (size_t)nt));
-  SDTFE_HIP(hipMemcpyAsync(t->pos4, pd, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice, s));
-  SDTFE_HIP(hipMemcpyAsync(t->tets, tetra, sizeof(int) * 8 * (size_t)nt, hipMemcpyHostToDevice, s));
+  SDTFE_RC(upload(t->pos4, pd, sizeof(double) * 4 * (size_t)n, s, st));
+  SDTFE_RC(upload(t->tets, tetra, sizeof(int) * 8 * (size_t)nt, s, st));
 
-  Scratch sc;
+  Scratch sc(s);
   int *bad = nullptr;
   SDTFE_RC(sc.get(&bad, 1));
   SDTFE_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
@@ -502,7 +506,7 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra) {
 int build_entry_grid(sdtfe_tess *t) {
   const int nt = t->nt;
   hipStream_t s = t->stream;
-  Scratch sc;
+  Scratch sc(s);
   unsigned char *mask = nullptr;
   int *cnt = nullptr, *off = nullptr, *tmp = nullptr;
   SDTFE_RC(sc.get(&mask, (size_t)nt));

@@ -86,7 +86,28 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
                      const sdtfe_map_params *params, double *const *rho_device,
                      const int *lanes_per_pixel, const int *tile_begin, const int *tile_count,
                      sdtfe_stats *item_stats, sdtfe_stats *total);
+// Stream-ordered device memory from the device's default pool, which keeps
+// freed blocks for reuse (capi.cpp): unlike hipMalloc / hipFree these do not
+// synchronise the device, so tessellations set up or freed on one stream do
+// not wait for kernels on others.
+hipError_t dev_malloc(void **p, size_t bytes, hipStream_t s);
+void dev_free(void *p, hipStream_t s);
+// Pinned double buffers for uploads from pageable memory by one host
+// thread: chunks are copied into one buffer while the other is in flight
+// (several threads then upload concurrently).
+struct Staging {
+  void *buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  size_t bytes = 0;
+};
+int staging_init(Staging &st, size_t bytes);
+void staging_free(Staging &st);
+int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st);
+// A tessellation as sdtfe_tess_create makes it, uploading through st.
+int tess_create_staged(const double *particle_data, int n, const int *tetra_data, int nt,
+                       float p_mass, Staging *st, sdtfe_tess **out);
 // field.hip
-int build_field(sdtfe_tess *t, const double *particle_data, const int *tetra_data);
+int build_field(sdtfe_tess *t, const double *particle_data, const int *tetra_data,
+                Staging *st);
 int build_entry_grid(sdtfe_tess *t);
 }  // namespace sdtfe

@@ -3,10 +3,11 @@
 //
 //   submit()      copies a request (particles, tets, map parameters) into
 //                 host memory owned by the server and queues it.
-//   setup thread  admits queued requests in arrival order while their HBM
+//   setup threads admit queued requests in arrival order while their HBM
 //                 footprint fits the budget and fewer than max_active are
-//                 resident, uploads them and builds their tessellations
-//                 (sdtfe_tess_create) on their own streams.
+//                 resident, upload them and build their tessellations
+//                 (sdtfe_tess_create) on their own streams, several requests
+//                 at a time (uploads overlap each other's kernels).
 //   walk thread   repeatedly launches one persistent batch over the next
 //                 tiles of every resident request (at most tile_budget tiles
 //                 per launch, shared round-robin), so a request admitted
@@ -73,7 +74,8 @@ struct sdtfe_server {
   int n_setup = 0;  // being set up (counted in used_bytes)
   bool stop = false;
   sdtfe_batch *batch = nullptr;
-  std::thread setup_thr, walk_thr;
+  std::vector<std::thread> setup_thr;
+  std::thread walk_thr;
   sdtfe::Clock::time_point t0;
   int64_t launches = 0;
   double walk_ms = 0;
@@ -102,15 +104,18 @@ void add_stats(sdtfe_stats &a, const sdtfe_stats &s) {
 void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status) {
   // caller holds no lock; device work of r is complete
   if (status == 0 && r->d_map) {
+    // on the request's own stream: a copy on the null stream would wait for
+    // every other request's setup
     const size_t npix = (size_t)r->params.grid_dim * r->params.grid_dim;
     std::vector<double> m(npix);
-    if (hipMemcpy(m.data(), r->d_map, sizeof(double) * npix, hipMemcpyDeviceToHost) !=
-        hipSuccess)
+    if (hipMemcpyAsync(m.data(), r->d_map, sizeof(double) * npix, hipMemcpyDeviceToHost,
+                       r->tess->stream) != hipSuccess ||
+        hipStreamSynchronize(r->tess->stream) != hipSuccess)
       status = SDTFE_EHIP;
     else
       for (size_t i = 0; i < npix; ++i) r->out[i] += m[i];
   }
-  if (r->d_map) (void)hipFree(r->d_map);
+  if (r->d_map) dev_free(r->d_map, r->tess->stream);
   r->d_map = nullptr;
   sdtfe_tess_destroy(r->tess);
   r->tess = nullptr;
@@ -126,6 +131,8 @@ void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status) {
 }
 
 void setup_loop(sdtfe_server *sv) {
+  Staging st;  // this thread's pinned upload buffers
+  if (staging_init(st, (size_t)8 << 20)) staging_free(st);  // unpinned uploads then
   for (;;) {
     std::shared_ptr<Request> r;
     {
@@ -139,19 +146,30 @@ void setup_loop(sdtfe_server *sv) {
                           (resident == 0);  // a lone request is always admitted
         return resident < sv->cfg.max_active && fits;
       });
-      if (sv->stop) return;
+      if (sv->stop) {
+        lk.unlock();
+        staging_free(st);
+        return;
+      }
       r = sv->queued.front();
       sv->queued.pop_front();
       sv->used_bytes += r->footprint;
       sv->n_setup++;
       r->t_admit = Clock::now();
     }
-    int rc = sdtfe_tess_create(r->pd.data(), (int)(r->pd.size() / 4), r->tets.data(),
-                               (int)(r->tets.size() / 8), r->p_mass, &r->tess);
+    int rc = tess_create_staged(r->pd.data(), (int)(r->pd.size() / 4), r->tets.data(),
+                                (int)(r->tets.size() / 8), r->p_mass, &st, &r->tess);
+    // the host copies are uploaded: release them now
+    std::vector<double>().swap(r->pd);
+    std::vector<int>().swap(r->tets);
     const size_t npix = (size_t)r->params.grid_dim * r->params.grid_dim;
-    if (rc == 0 && hipMalloc((void **)&r->d_map, sizeof(double) * npix) != hipSuccess)
+    if (rc == 0 &&
+        dev_malloc((void **)&r->d_map, sizeof(double) * npix, r->tess->stream) != hipSuccess)
       rc = fail(SDTFE_ENOMEM, "map buffer");
-    if (rc == 0 && hipMemset(r->d_map, 0, sizeof(double) * npix) != hipSuccess) rc = SDTFE_EHIP;
+    if (rc == 0 && (hipMemsetAsync(r->d_map, 0, sizeof(double) * npix, r->tess->stream) !=
+                        hipSuccess ||
+                    hipStreamSynchronize(r->tess->stream) != hipSuccess))
+      rc = SDTFE_EHIP;
     if (rc == 0) {
       r->lpp = sv->cfg.lanes_per_pixel > 0 ? sv->cfg.lanes_per_pixel
                                            : map_lanes(r->params.n_mc_samp);
@@ -281,7 +299,8 @@ int sdtfe_server_create(const sdtfe_server_config *cfg, sdtfe_server **out) {
     return rc;
   }
   sv->t0 = Clock::now();
-  sv->setup_thr = std::thread(setup_loop, sv);
+  const int n_setup = cfg->setup_threads > 0 ? std::min(cfg->setup_threads, 64) : 4;
+  for (int i = 0; i < n_setup; ++i) sv->setup_thr.emplace_back(setup_loop, sv);
   sv->walk_thr = std::thread(walk_loop, sv);
   *out = sv;
   return 0;
@@ -362,7 +381,8 @@ void sdtfe_server_destroy(sdtfe_server *sv) {
     sv->cv_setup.notify_all();
     sv->cv_walk.notify_all();
   }
-  if (sv->setup_thr.joinable()) sv->setup_thr.join();
+  for (std::thread &th : sv->setup_thr)
+    if (th.joinable()) th.join();
   if (sv->walk_thr.joinable()) sv->walk_thr.join();
   sdtfe_batch_destroy(sv->batch);
   delete sv;

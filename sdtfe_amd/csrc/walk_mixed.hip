@@ -167,13 +167,14 @@ extern const bool kHaveMixed = true;
 namespace {
 
 int ensure_defer(sdtfe_tess *t, size_t entries) {
-  if (!t->defer_count && hipMalloc((void **)&t->defer_count, sizeof(int)) != hipSuccess)
+  if (!t->defer_count &&
+      dev_malloc((void **)&t->defer_count, sizeof(int), t->stream) != hipSuccess)
     return fail(SDTFE_ENOMEM, "deferred-pixel counter allocation failed");
   if (entries <= t->defer_cap) return 0;
-  if (t->defer_buf) (void)hipFree(t->defer_buf);
+  dev_free(t->defer_buf, t->stream);
   t->defer_buf = nullptr;
   t->defer_cap = 0;
-  if (hipMalloc((void **)&t->defer_buf, sizeof(Deferred) * entries) != hipSuccess)
+  if (dev_malloc((void **)&t->defer_buf, sizeof(Deferred) * entries, t->stream) != hipSuccess)
     return fail(SDTFE_ENOMEM, "deferred-pixel queue allocation failed");
   t->defer_cap = entries;
   return 0;
