@@ -50,10 +50,18 @@ def test_mixed_rays_match_fp64(golden):
     # 4.2M sample-rays: rare near-edge cases (sides the FP32 filter leaves
     # open, on the first tet's faces too) occur many times
     ("clustered-large", 262144, 512, 16, 1.0, 0.5),
+    # the large-map layout at 1/512 of its size: a 32x32x16 lattice with
+    # Gaussian displacements (0.3 spacing) and 5% of the particles in halos
+    ("lattice", 32 * 32 * 16, 128, 8, 0.5, 0.25),
 ])
 def test_mixed_map_matches_fp64(kind, n, G, S, box_z, cz):
     _say(f"[gpu] mixed map {kind}")
-    pos = workloads.uniform(n, 7) if kind == "uniform" else workloads.clustered(n, seed=5)
+    if kind == "uniform":
+        pos = workloads.uniform(n, 7)
+    elif kind == "lattice":
+        pos = workloads.jittered_lattice(32, 32, 16, seed=5)
+    else:
+        pos = workloads.clustered(n, seed=5)
     tetra = triangulate(pos)
     kw = dict(seed=13, box_len_z=box_z, center=(0.5, 0.5, cz))
     with capi.Tessellation(to_particle_data(pos), tetra) as t:

@@ -213,3 +213,22 @@ def test_sample_split_maps_match_oracle(oracle, S):
     assert s32["tet_steps"] == st["tet_steps"]
     assert s32["samples"] == G * G * S
     assert _rel(r32, rho) <= 1e-5
+
+
+def test_lattice_map_matches_oracle(oracle):
+    """The large-map layout (jittered lattice plus halos, slab of half the
+    box height) at 32x32x16 particles: FP64 map, n_avg and tets against the
+    oracle."""
+    _say("[gpu] lattice-16384 96x96x8 vs oracle")
+    pos = workloads.jittered_lattice(32, 32, 16, seed=6)
+    tetra = triangulate(pos)
+    pd = to_particle_data(pos)
+    G, S, seed = 96, 8, 9
+    ref, st_o, navg_o = oracle.compute_density(pd.copy(), tetra, G, 1.0, 0.5, 1.0,
+                                               (0.5, 0.5, 0.25), S, 0.0, seed, want_navg=True)
+    with capi.Tessellation(pd, tetra) as t:
+        rho, st, navg = t.map(G, S, want_navg=True, seed=seed, box_len_z=0.5,
+                              center=(0.5, 0.5, 0.25))
+    np.testing.assert_array_equal(navg, navg_o)
+    assert _rel(rho, ref) <= RTOL_FP64
+    assert st["tet_steps"] == st_o["tet_steps"]
