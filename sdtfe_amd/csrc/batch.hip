@@ -56,13 +56,8 @@ struct BatchHdr {
 
 // Wave sub-tile shape for L lanes per pixel: 64/L pixels as bw x bh.
 SDTFE_HD inline void wave_block(int lpp, int &bw, int &bh) {
-  switch (lpp) {
-    case 1: bw = 8; bh = 8; break;
-    case 2: bw = 8; bh = 4; break;
-    case 4: bw = 4; bh = 4; break;
-    case 8: bw = 4; bh = 2; break;
-    default: bw = 2; bh = 2; break;  // 16
-  }
+  bw = wave_bw(lpp);
+  bh = wave_bh(lpp);
 }
 
 __global__ void batch_reset_kernel(const BatchHdr *hdr, int *counters,

@@ -78,17 +78,6 @@ struct LaneCounters {
   }
 };
 
-// Pixel of a lane: blocks of 256 lanes cover 16x16 pixels as four 8x8
-// wave tiles; block ids are remapped so neighbouring tiles share an XCD.
-__device__ inline void lane_pixel(int grid, int &px, int &py) {
-  const int nbx = (grid + 15) >> 4;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = tile % nbx, by = tile / nbx;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  px = bx * 16 + (w & 1) * 8 + (lane & 7);
-  py = by * 16 + (w >> 1) * 8 + (lane >> 3);
-}
-
 // v as hi + lo in FP32: hi = v rounded, lo = (v - hi) rounded (v - hi is
 // exact in FP64), so hi + lo is within 2^-48 |v| of v.
 SDTFE_HD inline void split32(double v, float &hi, float &lo) {
@@ -114,7 +103,8 @@ SDTFE_HD constexpr int map_lanes(int samples) {
 SDTFE_HD constexpr int map_window_words(int L) { return L == 4 ? 4 : 1; }
 
 // Pixel and sample slice j (0..L-1) of a lane with L lanes per pixel:
-// blocks of 256 lanes cover 2 x 2 wave tiles, remapped as in lane_pixel.
+// blocks of 256 lanes cover 2 x 2 wave tiles; block ids are remapped so
+// neighbouring tiles share an XCD.
 template <int L>
 __device__ inline void split_pixel(int grid, int &px, int &py, int &j) {
   constexpr int bw = wave_bw(L), bh = wave_bh(L);
