@@ -184,10 +184,11 @@ void staging_free(Staging &st) {
     st.ev[k] = nullptr;
     st.buf[k] = nullptr;
   }
+  st.bytes = 0;  // uploads then go straight from pageable memory
 }
 
 int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st) {
-  if (!st || !st->bytes) {
+  if (!st || !st->bytes || !st->buf[0] || !st->buf[1]) {
     SDTFE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
     return 0;
   }

@@ -55,8 +55,9 @@ struct Request {
 
 int64_t estimate_footprint(int n, int nt, int grid) {
   // resident: pos4 32 B/vertex; per tet: row 32, volume 8, gradient 24,
-  // records 144 + 64 + 16; map 8 B/pixel; setup scratch ~ 16 B/tet + 12 B/vertex
-  return (int64_t)n * 44 + (int64_t)nt * 304 + (int64_t)grid * grid * 8 + (1 << 20);
+  // records 144 (FP64) + 96 (mixed) + 4 x 32 (faces); map 8 B/pixel; setup
+  // scratch ~ 16 B/tet + 12 B/vertex
+  return (int64_t)n * 44 + (int64_t)nt * 448 + (int64_t)grid * grid * 8 + (1 << 20);
 }
 
 }  // namespace

@@ -501,7 +501,7 @@ SDTFE_HD inline double walk_mixed_resume(const Rec32 *__restrict__ rec32,
                                    cnt, rs))
     return v;
   resumed = true;
-  if (rs.enter < 0) return walk64(rec64, entry, x, y, g, key, crossed, cnt);
+  if (rs.enter < 0 || rs.steps == 0) return walk64(rec64, entry, x, y, g, key, crossed, cnt);
   Tet64 q;
   load_tet(rec64, rs.t, q);
   int c;
