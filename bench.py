@@ -147,13 +147,20 @@ def cpu_baseline(pos, tetra, cfg, seed, target_s, gpu_map):
 
 def tessellation(name, pos, cache):
     """Host Qhull tessellation, optionally cached in an .npz (e.g. under /tmp
-    between profiling passes of one session)."""
+    between profiling passes of one session).  Above 2M particles the
+    Delaunay tetrahedralisation is assembled from x-slabs in parallel (one
+    Qhull run over 8M particles takes many minutes on one core)."""
     from sdtfe_amd.tessellate import triangulate
+    from sdtfe_amd.tessellate_chunked import triangulate_chunked
     if cache and os.path.exists(cache):
         with np.load(cache) as z:
             if z["n"] == pos.shape[0] and str(z["name"]) == name:
                 return z["tetra"]
-    tetra = triangulate(pos)
+    if pos.shape[0] > 2_000_000:
+        workers = min(16, os.cpu_count() or 1)
+        tetra = triangulate_chunked(pos, slabs=16, workers=workers, log=log)
+    else:
+        tetra = triangulate(pos)
     if cache:
         np.savez(cache, tetra=tetra, n=pos.shape[0], name=name)
     return tetra
@@ -298,7 +305,7 @@ def main():
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the path shards by "
-              "whole maps or 4-row bands, which this build does not run across GPUs.",
+              "whole maps or 4-row bands, which this library does not run across GPUs.",
               file=sys.stderr)
         sys.exit(2)
 

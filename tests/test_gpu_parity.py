@@ -232,3 +232,33 @@ def test_lattice_map_matches_oracle(oracle):
     np.testing.assert_array_equal(navg, navg_o)
     assert _rel(rho, ref) <= RTOL_FP64
     assert st["tet_steps"] == st_o["tet_steps"]
+
+
+@pytest.mark.parametrize("G,S", [(1, 3), (7, 0), (7, 5), (50, 20)])
+def test_small_and_ragged_maps(oracle, G, S):
+    """Grids of 1 and 7 pixels and one not divisible by 4, and no samples
+    at all.  The reference walks 4x4 tiles (src/dtfe.c:601-608), so it
+    covers the first 4 floor(G / 4) rows and columns; there the FP64 map
+    equals the oracle's, and everywhere the mixed map matches the FP64 one.
+    With no samples every pixel receives 0 (src/dtfe.c:639-645)."""
+    _say(f"[gpu] ragged map {G}x{G}x{S}")
+    pos = workloads.clustered(4096, seed=8)
+    tetra = triangulate(pos)
+    pd = to_particle_data(pos)
+    ref, _, navg_o = oracle.compute_density(pd.copy(), tetra, G, 1.0, 1.0, 1.0,
+                                            (0.5, 0.5, 0.5), S, 0.0, 4, want_navg=True)
+    with capi.Tessellation(pd, tetra) as t:
+        rho, st, navg = t.map(G, S, want_navg=True, seed=4)
+        r32, s32, n32 = t.map(G, S, want_navg=True, seed=4, precision=capi.PREC_MIXED)
+    assert st["samples"] == G * G * S and s32["samples"] == G * G * S
+    c = 4 * (G // 4)
+    cov = np.zeros((G, G), dtype=bool)
+    cov[:c, :c] = True
+    cov = cov.ravel()
+    np.testing.assert_array_equal(navg[cov], navg_o[cov])
+    assert _rel(rho[cov], ref[cov]) <= RTOL_FP64
+    np.testing.assert_array_equal(n32, navg)
+    assert s32["tet_steps"] == st["tet_steps"]
+    assert np.isfinite(rho).all() and _rel(r32, rho) <= 1e-5
+    if S == 0:
+        assert not rho.any() and not r32.any() and not navg.any()

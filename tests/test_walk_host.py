@@ -1,6 +1,7 @@
 """The device walk code (walk_fp64.hip / walk_fp64_core.h) compiled for the
 host and run on the golden rays: pins the kernel's logic without a GPU.
-(The product never runs this build; the GPU tests check the device build.)"""
+(The library never runs this host build; the GPU tests check the device
+build.)"""
 import os
 import shutil
 import subprocess
