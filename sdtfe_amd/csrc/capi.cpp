@@ -215,7 +215,7 @@ void free_tess(sdtfe_tess *t) {
   void *bufs[] = {t->pos4,    t->tets,     t->vol,     t->nabla,   t->rec64,
                   t->rec32,   t->facerec,   t->tri_xy,   t->tri_tet, t->tri_abc, t->cell_off,
                   t->cell_tri, t->stats,   t->map_buf, t->navg_buf, t->defer_buf,
-                  t->defer_count};
+                  t->defer_count, t->dq_off, t->dq_tmp, t->dq_val, t->dq_ok};
   if (t->stream) {
     (void)hipStreamSynchronize(t->stream);
     for (void *b : bufs) dev_free(b, t->stream);

@@ -70,6 +70,13 @@ struct sdtfe_tess {
   sdtfe::Deferred *defer_buf = nullptr;  // pixels queued for the FP64 pass
   size_t defer_cap = 0;
   int *defer_count = nullptr;
+  // second pass, one lane per queued sample: per-entry counts and offsets
+  // (+ scan scratch), per-sample values and flags
+  int *dq_off = nullptr, *dq_tmp = nullptr;
+  size_t dq_off_cap = 0;
+  double *dq_val = nullptr;
+  unsigned char *dq_ok = nullptr;
+  size_t dq_val_cap = 0;
   int64_t device_bytes = 0;
   double t_setup_ms = 0;
 };

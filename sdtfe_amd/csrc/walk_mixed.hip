@@ -2,6 +2,8 @@
 // (walk_mixed_core.h).  A map is two passes: the mixed walk over every
 // pixel, then the FP64 walk over the queued pixels' handed-over samples
 // (walk_defer.h).
+#include <cstdlib>
+
 #include "walk_defer.h"
 
 namespace sdtfe {
@@ -130,6 +132,68 @@ __global__ __launch_bounds__(256) void map_deferred_kernel(const Rec32 *__restri
   cnt.flush(stats);
 }
 
+// Second pass with one lane per queued sample, for a queue longer than the
+// chip holds waves (most queued pixels hand over one or two samples, so a
+// wave per pixel would leave most lanes idle): per-entry counts, their
+// scan, the walks, then each entry's values added in sample order (the same
+// additions as map_deferred_kernel's, so results are identical).
+__global__ void defer_count_kernel(const Deferred *__restrict__ q, int n, int samples,
+                                   int *cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = deferred_count(q + i, samples);
+}
+
+template <bool kWide>
+__global__ __launch_bounds__(256) void defer_walk_kernel(const Rec32 *__restrict__ rec32,
+                                                        const Rec64 *__restrict__ rec64,
+                                                        const FaceRec *__restrict__ frec,
+                                                        EntryGrid eg, MapGeom g, MixedConsts mc,
+                                                        const Deferred *__restrict__ q, int n,
+                                                        const int *__restrict__ off, int total,
+                                                        double *val, unsigned char *ok,
+                                                        unsigned long long *stats) {
+  __shared__ uint8_t faces[kFaceCodes];
+  load_faces(faces);
+  const DeferCtx c{rec32, rec64, frec, faces, eg, g, mc.flt};
+  LaneCounters cnt;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) {
+    int lo = 0, hi = n - 1;  // the entry: the last e with off[e] <= i
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= i)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    bool k_ok;
+    val[i] = deferred_walk<kWide>(q + lo, deferred_sample(q + lo, i - off[lo]), c, k_ok, cnt);
+    ok[i] = (unsigned char)k_ok;
+  }
+  cnt.flush(stats);
+}
+
+__global__ void defer_combine_kernel(const Deferred *__restrict__ q, int n,
+                                     const int *__restrict__ off, const double *__restrict__ val,
+                                     const unsigned char *__restrict__ ok, double *out, int *navg,
+                                     int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sum = q[i].sum;
+  int n_ok = q[i].n_ok;
+  for (int k = off[i]; k < off[i + 1]; ++k) {
+    sum += val[k];
+    n_ok += ok[k];
+  }
+  const uint64_t pix = q[i].pix;
+  const double v = n_ok > 0 ? sum / n_ok : 0.0;
+  if (accumulate)
+    out[pix] += v;
+  else
+    out[pix] = v;
+  if (navg) navg[pix] = n_ok;
+}
+
 // Rays one by one; a handed-over ray goes on in the FP64 walk in place
 // (this kernel serves inspection, not maps).
 __global__ void trace_mixed_kernel(const Rec32 *__restrict__ rec32,
@@ -180,6 +244,64 @@ int ensure_defer(sdtfe_tess *t, size_t entries) {
   return 0;
 }
 
+// Queues up to this long get a wave per pixel (1024 blocks of 4 waves hold
+// them all at once); longer ones a lane per sample.  SDTFE_DEFER_WAVE_ENTRIES
+// overrides the limit (both forms give the same map; tests run each).
+constexpr int kDeferWaveEntries = 4096;
+
+int defer_wave_entries() {
+  const char *e = getenv("SDTFE_DEFER_WAVE_ENTRIES");
+  return e && *e ? atoi(e) : kDeferWaveEntries;
+}
+
+int defer_by_sample(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, double *out,
+                    int *navg, int accumulate, int n, bool wide) {
+  hipStream_t s = t->stream;
+  if ((size_t)n + 1 > t->dq_off_cap) {
+    dev_free(t->dq_off, s);
+    dev_free(t->dq_tmp, s);
+    t->dq_off = t->dq_tmp = nullptr;
+    t->dq_off_cap = 0;
+    if (dev_malloc((void **)&t->dq_off, sizeof(int) * ((size_t)n + 1), s) != hipSuccess ||
+        dev_malloc((void **)&t->dq_tmp, sizeof(int) * scan_tmp_ints((int64_t)n + 1), s) !=
+            hipSuccess)
+      return fail(SDTFE_ENOMEM, "second-pass offsets allocation failed");
+    t->dq_off_cap = (size_t)n + 1;
+  }
+  // counts in dq_off[0, n), dq_off[n] = 0, scanned in place into offsets
+  // (dq_off[n] = the number of samples)
+  defer_count_kernel<<<(n + 255) / 256, 256, 0, s>>>(t->defer_buf, n, g.samples, t->dq_off);
+  SDTFE_HIP(hipMemsetAsync(t->dq_off + n, 0, sizeof(int), s));
+  int total = 0;
+  int rc = scan_exclusive(t->dq_off, t->dq_off, (int64_t)n + 1, t->dq_tmp, &total, s);
+  if (rc) return rc;
+  if (total <= 0) return fail(SDTFE_EHIP, "second pass: queued pixels list no samples");
+  if ((size_t)total > t->dq_val_cap) {
+    dev_free(t->dq_val, s);
+    dev_free(t->dq_ok, s);
+    t->dq_val = nullptr;
+    t->dq_ok = nullptr;
+    t->dq_val_cap = 0;
+    if (dev_malloc((void **)&t->dq_val, sizeof(double) * (size_t)total, s) != hipSuccess ||
+        dev_malloc((void **)&t->dq_ok, (size_t)total, s) != hipSuccess)
+      return fail(SDTFE_ENOMEM, "second-pass values allocation failed");
+    t->dq_val_cap = (size_t)total;
+  }
+  const unsigned nb = (unsigned)((total + 255) / 256);
+  if (wide)
+    defer_walk_kernel<true><<<nb, 256, 0, s>>>(t->rec32, t->rec64, t->facerec, t->grid, g, mc,
+                                               t->defer_buf, n, t->dq_off, total, t->dq_val,
+                                               t->dq_ok, t->stats);
+  else
+    defer_walk_kernel<false><<<nb, 256, 0, s>>>(t->rec32, t->rec64, t->facerec, t->grid, g, mc,
+                                                t->defer_buf, n, t->dq_off, total, t->dq_val,
+                                                t->dq_ok, t->stats);
+  defer_combine_kernel<<<(n + 255) / 256, 256, 0, s>>>(t->defer_buf, n, t->dq_off, t->dq_val,
+                                                       t->dq_ok, out, navg, accumulate);
+  SDTFE_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace
 
 template <int L>
@@ -213,7 +335,8 @@ void launch_first_pass(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, d
 
 // The queue holds a quarter of the pixels; a map that overflows it is run
 // again with room for all of them (an accumulating map gets that room from
-// the start, since its first pass cannot be repeated).
+// the start, since its first pass cannot be repeated).  The second pass
+// starts once the queue's length is known.
 int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
   const size_t npix = (size_t)g.grid * (size_t)g.grid;
   if (npix > (size_t)INT_MAX) return fail(SDTFE_EINVAL, "map too large for the mixed walk");
@@ -221,28 +344,36 @@ int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
   int rc = ensure_defer(t, want);
   if (rc) return rc;
   const MixedConsts mc = mixed_consts(t, g);
+  const bool wide = (int64_t)t->nt > kFaceRecNarrowTets;
   for (;;) {
     const DeferQueue dq{t->defer_buf, t->defer_count, (int)std::min(t->defer_cap, npix)};
     defer_reset_kernel<<<1, 1, 0, t->stream>>>(t->defer_count);
     SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
     launch_first_pass(t, g, mc, out, navg, accumulate, dq);
     SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
-    if ((int64_t)t->nt > kFaceRecNarrowTets)
+    SDTFE_HIP(hipGetLastError());
+    int count = 0;
+    SDTFE_HIP(hipMemcpyAsync(&count, t->defer_count, sizeof(int), hipMemcpyDeviceToHost,
+                             t->stream));
+    SDTFE_HIP(hipStreamSynchronize(t->stream));
+    if (count > dq.cap) {
+      // overflow: the queued pixels beyond the queue were not written; the
+      // repeat writes every pixel again
+      SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
+      if ((rc = ensure_defer(t, npix))) return rc;
+      continue;
+    }
+    if (count == 0) return 0;
+    if (count > defer_wave_entries())
+      return defer_by_sample(t, g, mc, out, navg, accumulate, count, wide);
+    if (wide)
       map_deferred_kernel<true><<<1024, 256, 0, t->stream>>>(
           t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
     else
       map_deferred_kernel<false><<<1024, 256, 0, t->stream>>>(
           t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
     SDTFE_HIP(hipGetLastError());
-    int count = 0;
-    SDTFE_HIP(hipMemcpyAsync(&count, t->defer_count, sizeof(int), hipMemcpyDeviceToHost,
-                             t->stream));
-    SDTFE_HIP(hipStreamSynchronize(t->stream));
-    if (count <= dq.cap) return 0;
-    // overflow: the queued pixels beyond the queue were not written; every
-    // other pixel is written again by the repeat
-    SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
-    if ((rc = ensure_defer(t, npix))) return rc;
+    return 0;
   }
 }
 

@@ -5,6 +5,7 @@ tets crossed, perturbations and n_avg must be identical; values carry only
 the FP32 integrand's rounding.  Tolerance: 1e-5 relative per pixel (and per
 ray), the bound the mixed form is held to.
 """
+import os
 import sys
 
 import numpy as np
@@ -91,3 +92,30 @@ def test_mixed_map_is_deterministic():
     np.testing.assert_array_equal(na, nb)
     counts = lambda st: {k: v for k, v in st.items() if not k.startswith("t_")}  # noqa: E731
     assert counts(sa) == counts(sb)
+
+
+def test_second_pass_forms_agree():
+    """The second pass as a wave per queued pixel and as a lane per queued
+    sample (the form for long queues, forced here by a limit of 0 queued
+    pixels) gives the same map, n_avg and counters bit for bit."""
+    _say("[gpu] mixed map, second pass per pixel vs per sample")
+    pos = workloads.clustered(262144, seed=5)
+    tetra = triangulate(pos)
+    kw = dict(seed=13, want_navg=True, precision=capi.PREC_MIXED)
+    with capi.Tessellation(to_particle_data(pos), tetra) as t:
+        res = {}
+        for lim in ("", "0"):
+            os.environ["SDTFE_DEFER_WAVE_ENTRIES"] = lim
+            try:
+                res[lim] = t.map(512, 16, **kw)
+            finally:
+                os.environ.pop("SDTFE_DEFER_WAVE_ENTRIES", None)
+        r64, _ = t.map(512, 16, seed=13)
+    (a, sa, na), (b, sb, nb) = res[""], res["0"]
+    _say(f"  handed to fp64: {sa['fp64_redo']} of {sa['samples']} samples")
+    assert sa["fp64_redo"] > 0
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(na, nb)
+    counts = lambda st: {k: v for k, v in st.items() if not k.startswith("t_")}  # noqa: E731
+    assert counts(sa) == counts(sb)
+    assert _relerr(b, r64).max() <= RTOL_MIXED
