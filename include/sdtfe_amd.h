@@ -9,6 +9,8 @@
  *   compute_density   replaces src/dtfe.h:13-15 (src/dtfe.c:554-654)
  *   pre_compute_vol   replaces src/triangulate.h:26 (src/triangulate.c:183-225)
  *   gradients         replaces src/triangulate.h:27 (src/triangulate.c:233-260)
+ *   interpolate_to_3d_grid  replaces src/dtfe.h:17-18 (src/dtfe.c:667-735)
+ *   compute_3d_density      replaces src/dtfe.h:20-21 (src/dtfe.c:657-665)
  *
  * Behaviour kept from the reference: particle_data is N x 4 float64 AoS
  * (x, y, z, w) and column 3 is overwritten with the vertex density;
@@ -57,6 +59,26 @@ void pre_compute_vol(double **particle_data, int n_particles, int *tetra_data, i
 /* src/triangulate.h:27: returns a malloc'd n_tetra x 3 array (free() it),
  * the per-tet gradient of the vertex values in particle_data[:,3]. */
 double *gradients(int *tetra_data, int n_tetra, double *particle_data);
+
+/* src/dtfe.h:17-18: the vertex field in particle_data[:,3] sampled on a
+ * grid_dim^3 grid, grid[(i * grid_dim + j) * grid_dim + k] for cell (x i,
+ * y j, z k) of the cube of side box_len about center, ACCUMULATED into: each
+ * cell adds 1/supersampling^3 times the field at supersampling^3 points
+ * (supersampling x supersampling vertical rays per column, supersampling
+ * points per cell along each).  FP64, the reference's arithmetic; the
+ * perturbation's random corners come from the counter-based stream (seed as
+ * for compute_density). */
+void interpolate_to_3d_grid(double *particle_data, int n_particles, int *tetra_data,
+                            int n_tetra, int grid_dim, double box_len, double *grid,
+                            double center_x, double center_y, double center_z,
+                            unsigned supersampling);
+
+/* src/dtfe.h:20-21: pre_compute_vol (column 3 <- density), then
+ * interpolate_to_3d_grid of the density into rho. */
+void compute_3d_density(double *particle_data, int n_particles, int *tetra_data, int n_tetra,
+                        int grid_dim, double box_len, double *rho, float p_mass,
+                        double center_x, double center_y, double center_z,
+                        unsigned supersampling);
 
 /* ---------------------------------------------------------- extended */
 
@@ -156,6 +178,27 @@ int sdtfe_map(sdtfe_tess *t, const sdtfe_map_params *p, double *rho, int *n_avg,
 int sdtfe_map_device(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_device,
                      sdtfe_stats *stats);
 int sdtfe_tess_sync(sdtfe_tess *t);
+
+/* A volumetric grid of the tessellation's vertex field (the densities, or
+ * the field given to sdtfe_tess_create_field), as interpolate_to_3d_grid
+ * computes it.  Ray (is, js) of column c = i * grid_dim + j draws its
+ * perturbation corners from (seed, c, is * supersampling + js). */
+typedef struct {
+  int grid_dim;            /* cells per side (any size >= 1) */
+  unsigned supersampling;  /* >= 1 */
+  double box_len;          /* the cube's side; rays run its full height */
+  double center[3];
+  uint64_t seed;
+  int max_steps;           /* per-ray tet bound, <= 0 for the default */
+} sdtfe_volume_params;
+
+/* grid (host, grid_dim^3 doubles) is accumulated into. */
+int sdtfe_volume(sdtfe_tess *t, const sdtfe_volume_params *p, double *grid,
+                 sdtfe_stats *stats);
+/* Into device memory (accumulated into), asynchronous on the
+ * tessellation's stream; with stats != NULL it waits and fills them. */
+int sdtfe_volume_device(sdtfe_tess *t, const sdtfe_volume_params *p, double *grid_device,
+                        sdtfe_stats *stats);
 
 /* ------------------------------------------------------------ batches */
 

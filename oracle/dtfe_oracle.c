@@ -554,3 +554,171 @@ void sdo_compute_density(double *pd, int n, const int *tet, int nt, int G, doubl
   free(tri);
   free(nabla);
 }
+
+/* ------------------------------------------------------ volumetric (F2) */
+
+int sdo_ray_3d(int entry, const int *tet, const double *nabla, const double *pd, double x,
+               double y, double off_z, double lz, int nz, double dxy, unsigned ss,
+               uint64_t seed, uint64_t pixel, uint32_t sample, double *rho_vector,
+               sdo_stats *st) {
+  /* evaluate_3d_ray_on_grid, src/dtfe.c:376-516: the walk of shoot_3d_ray;
+   * in each tet crossed the fine z points offset_z + dzs*(k + 1/2) with
+   * enter_z <= z < leave_z (k clamped to [0, nz*ss)) add ss_weight times
+   * the tet's linear field to cell k / ss.  A failed walk returns 0 and
+   * keeps what it added (:444-449, 462-472). */
+  int crossed = 0;
+  if (entry == -1) return 0;
+  const double dz = lz / nz;           /* :401 */
+  const double dzs = dz / ss;          /* :403 */
+  const int nzs = nz * ss;             /* :404 */
+  const double ss_weight = 1. / (ss * ss * ss); /* :406 */
+  const double max_dist = dxy / 2.0;   /* :408 */
+  const double step = max_dist / SDO_PERTURB_MAX;
+  double ox = x, oy = y;
+  uint32_t draw = 2;
+  const int *row = NULL;
+  int cur = entry, prev = -1;
+  int f_in = -1, f_out = 0;
+  for (;;) {
+    int nxt = row ? row[4 + f_out] : entry;
+    if (nxt == -1) break;
+    if (nxt == prev) { /* turn around, :433-439 */
+      int t = f_in;
+      f_in = f_out;
+      f_out = t;
+      nxt = row[4 + f_out];
+      if (nxt == -1) break;
+    }
+    prev = cur;
+    cur = nxt;
+    row = tet + 8 * (size_t)cur;
+    const double *v[4] = {pd + 4 * (size_t)row[0], pd + 4 * (size_t)row[1],
+                          pd + 4 * (size_t)row[2], pd + 4 * (size_t)row[3]};
+    double z_in = 0, z_out = 0;
+    int special = 0;
+    if (!sdo_plucker(lz, ox, oy, v, &f_in, &f_out, &z_in, &z_out, &special)) {
+      if (st) st->no_intersection++;
+      return 0;
+    }
+    int tries = 0;
+    while (special) { /* perturb_ray, src/dtfe.c:219-245 */
+      int r = (int)(sdo_draw31(seed, pixel, sample, draw++) % 4u);
+      double d0 = v[r][0] - ox, d1 = v[r][1] - oy;
+      double mag = sqrt(d0 * d0 + d1 * d1);
+      if (mag > step) {
+        d0 = d0 / mag;
+        d1 = d1 / mag;
+        d0 = d0 * step;
+        d1 = d1 * step;
+      }
+      ox = ox + d0;
+      oy = oy + d1;
+      if (st) st->perturb_steps++;
+      if (++tries > SDO_PERTURB_MAX) {
+        if (st) st->perturb_exhausted++;
+        return 0;
+      }
+      if (!sdo_plucker(lz, ox, oy, v, &f_in, &f_out, &z_in, &z_out, &special)) {
+        if (st) st->no_isect_perturb++;
+        return 0;
+      }
+    }
+    ++crossed;
+    /* fine points of this tet, :482-513 */
+    int e = (int)ceil((z_in - off_z - dzs / 2.) / dzs);
+    e = e < 0 ? 0 : e;
+    int l = (int)ceil((z_out - off_z - dzs / 2.) / dzs);
+    l = l > nzs ? nzs : l;
+    if (l > e) {
+      const double *a = v[0];
+      const double *g = nabla + 3 * (size_t)cur;
+      double fixed = a[3] + g[0] * (ox - a[0]) + g[1] * (oy - a[1]) - g[2] * a[2];
+      for (int k = e; k < l; ++k) {
+        double z = off_z + dzs * k + dzs / 2.;
+        rho_vector[k / (int)ss] += (fixed + g[2] * z) * ss_weight;
+      }
+    }
+  }
+  return crossed;
+}
+
+void sdo_volume_columns(const double *pd, const int *tet, const double *nabla, const int *tri,
+                        int G, double box_len, double *grid, double cx, double cy, double cz,
+                        unsigned ss, uint64_t seed, const int *entries, int n_threads,
+                        sdo_stats *stats) {
+  /* interpolate_to_3d_grid's loop, src/dtfe.c:686-731: column (i, j) at x
+   * index i, y index j owns grid[(i*G + j)*G ...]; its ss x ss rays, in
+   * (is, js) order, add into it.  Columns are independent, so the result
+   * does not depend on the thread count. */
+  const double dxy = box_len / G;         /* :688 */
+  const double off_x = -(box_len / 2.0) + cx;
+  const double off_y = -(box_len / 2.0) + cy;
+  const double off_z = -(box_len / 2.0) + cz;
+  const double ds = dxy / ss;             /* :695 */
+  const int64_t n_col = (int64_t)G * G;
+  sdo_stats total;
+  memset(&total, 0, sizeof total);
+#if defined(_OPENMP)
+  if (n_threads <= 0) n_threads = omp_get_max_threads();
+#pragma omp parallel num_threads(n_threads)
+#endif
+  {
+    sdo_stats loc;
+    memset(&loc, 0, sizeof loc);
+#if defined(_OPENMP)
+#pragma omp for schedule(dynamic, 16)
+#endif
+    for (int64_t c = 0; c < n_col; ++c) {
+      const int i = (int)(c / G), j = (int)(c % G);
+      double *col = grid + (size_t)c * (size_t)G;
+      for (int is = 0; is < (int)ss; ++is)
+        for (int js = 0; js < (int)ss; ++js) {
+          double x = i * dxy + off_x + is * ds + ds / 2.;
+          double y = j * dxy + off_y + js * ds + ds / 2.;
+          const uint32_t r = (uint32_t)(is * (int)ss + js);
+          loc.samples++;
+          int ft = entries ? entries[(size_t)c * ss * ss + r] : sdo_pt_loc_2d(tri, pd, x, y, 1 << 24);
+          if (ft < 0) {
+            loc.missed++;
+            continue;
+          }
+          int crossed = sdo_ray_3d(ft, tet, nabla, pd, x, y, off_z, box_len, G, dxy, ss, seed,
+                                   (uint64_t)c, r, col, &loc);
+          loc.tet_steps += crossed;
+          if (crossed == 0) loc.failed++;
+        }
+    }
+#if defined(_OPENMP)
+#pragma omp critical
+#endif
+    sdo_stats_add(&total, &loc);
+  }
+  if (stats) sdo_stats_add(stats, &total);
+}
+
+void sdo_interpolate_3d(const double *pd, int n, const int *tet, int nt, int G, double box_len,
+                        double *grid, double cx, double cy, double cz, unsigned ss,
+                        uint64_t seed, int n_threads, sdo_stats *stats) {
+  /* interpolate_to_3d_grid, src/dtfe.c:667-735 */
+  (void)n;
+  double *nabla = (double *)malloc(sizeof(double) * 3 * (size_t)(nt > 0 ? nt : 1));
+  sdo_gradients(tet, nt, pd, nabla);
+  unsigned char *mask = (unsigned char *)malloc((size_t)(nt > 0 ? nt : 1));
+  int nf = sdo_forward_faces(tet, nt, pd, mask);
+  int *tri = (int *)malloc(sizeof(int) * 8 * (size_t)(nf > 0 ? nf : 1));
+  sdo_copy_fh_triangles(tet, nt, mask, nf, tri);
+  free(mask);
+  if (nf > 0)
+    sdo_volume_columns(pd, tet, nabla, tri, G, box_len, grid, cx, cy, cz, ss, seed, NULL,
+                       n_threads, stats);
+  free(tri);
+  free(nabla);
+}
+
+void sdo_compute_3d_density(double *pd, int n, const int *tet, int nt, int G, double box_len,
+                            double *rho, float p_mass, double cx, double cy, double cz,
+                            unsigned ss, uint64_t seed, int n_threads, sdo_stats *stats) {
+  /* compute_3d_density, src/dtfe.c:657-665 */
+  sdo_pre_compute_vol(pd, n, tet, nt, p_mass);
+  sdo_interpolate_3d(pd, n, tet, nt, G, box_len, rho, cx, cy, cz, ss, seed, n_threads, stats);
+}

@@ -108,6 +108,45 @@ int64_t sdo_map_bands(const double *particle_data, const int *tetra_data,
                       int band_begin, int band_step, int n_threads,
                       sdo_stats *stats, int *n_avg_out);
 
+/* F2  src/dtfe.c:376-516 -- one ray of the volumetric path: in each tet
+ * crossed, adds ss_weight = 1/ss^3 times the linear field at the fine z
+ * points offset_z + dzs*(k + 1/2) (dzs = lz/nz/ss, enter_z <= z < leave_z)
+ * into rho_vector[k / ss].  (pixel, sample) key the perturbation draws,
+ * which start at draw 2.  Returns tets crossed (0 on failure, when what was
+ * added stays, as in the reference). */
+int sdo_ray_3d(int entry, const int *tetra_data, const double *nabla,
+               const double *particle_data, double x, double y, double offset_z,
+               double lz, int nz, double delta_xy, unsigned supersampling,
+               uint64_t seed, uint64_t pixel, uint32_t sample, double *rho_vector,
+               sdo_stats *stats);
+
+/* The column loop of interpolate_to_3d_grid (src/dtfe.c:686-731) given the
+ * field (particle_data[:,3]), nabla and triangle list.  grid is G^3 in
+ * [x][y][z] order and is accumulated into.  Ray (is, js) of column
+ * c = i*G + j is keyed (seed, c, is*ss + js).  entries (nullable): entry tet
+ * per ray in (c, is, js) order instead of pt_loc_2d. */
+void sdo_volume_columns(const double *particle_data, const int *tetra_data,
+                        const double *nabla, const int *tri, int grid_dim,
+                        double box_len, double *grid, double center_x,
+                        double center_y, double center_z, unsigned supersampling,
+                        uint64_t seed, const int *entries, int n_threads,
+                        sdo_stats *stats);
+
+/* interpolate_to_3d_grid (src/dtfe.c:667-735) with the keyed stream. */
+void sdo_interpolate_3d(const double *particle_data, int n_particles,
+                        const int *tetra_data, int n_tetra, int grid_dim,
+                        double box_len, double *grid, double center_x,
+                        double center_y, double center_z, unsigned supersampling,
+                        uint64_t seed, int n_threads, sdo_stats *stats);
+
+/* compute_3d_density (src/dtfe.c:657-665): pre_compute_vol, then the above. */
+void sdo_compute_3d_density(double *particle_data, int n_particles,
+                            const int *tetra_data, int n_tetra, int grid_dim,
+                            double box_len, double *rho, float p_mass,
+                            double center_x, double center_y, double center_z,
+                            unsigned supersampling, uint64_t seed, int n_threads,
+                            sdo_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

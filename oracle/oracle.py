@@ -75,6 +75,20 @@ def lib():
                                     C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
                                     C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(Stats),
                                     C.c_void_p]
+        L.sdo_ray_3d.restype = C.c_int
+        L.sdo_ray_3d.argtypes = [C.c_int, _ip, _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, C.c_int, C.c_double, C.c_uint, C.c_uint64,
+                                 C.c_uint64, C.c_uint32, _dp, C.POINTER(Stats)]
+        L.sdo_volume_columns.argtypes = [_dp, _ip, _dp, _ip, C.c_int, C.c_double, _dp,
+                                         C.c_double, C.c_double, C.c_double, C.c_uint,
+                                         C.c_uint64, C.c_void_p, C.c_int, C.POINTER(Stats)]
+        L.sdo_interpolate_3d.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double, _dp,
+                                         C.c_double, C.c_double, C.c_double, C.c_uint,
+                                         C.c_uint64, C.c_int, C.POINTER(Stats)]
+        L.sdo_compute_3d_density.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double,
+                                             _dp, C.c_float, C.c_double, C.c_double,
+                                             C.c_double, C.c_uint, C.c_uint64, C.c_int,
+                                             C.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -115,6 +129,16 @@ def ref():
         R.rayTetraPlucker_full.restype = C.c_int
         R.rayTetraPlucker_full.argtypes = [_dp, C.POINTER(C.c_void_p * 4), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), _dp, _dp, C.POINTER(C.c_int)]
+        R.refh_rand_calls.restype = C.c_int64
+        R.interpolate_to_3d_grid.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double,
+                                             _dp, C.c_double, C.c_double, C.c_double, C.c_uint]
+        R.compute_3d_density.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double, _dp,
+                                         C.c_float, C.c_double, C.c_double, C.c_double,
+                                         C.c_uint]
+        R.evaluate_3d_ray_on_grid.argtypes = [C.c_int, _ip, C.c_void_p, _dp, C.c_double,
+                                              C.c_double, C.c_double, C.c_double, C.c_int,
+                                              C.c_double, C.POINTER(C.c_int), C.c_uint,
+                                              C.c_void_p]
         R.rotate3d.argtypes = [_dp, C.c_size_t, _dp, _dp]
         R.convert_to_qhdata.restype = C.c_void_p
         R.convert_to_qhdata.argtypes = [np.ctypeslib.ndpointer(dtype=np.float32,
@@ -200,3 +224,58 @@ def map_bands(pd, tetra, nabla, tri, grid_dim, box_len, box_len_z, center, n_sam
                             center[0], center[1], center[2], n_samp, delta_sample, seed,
                             band_begin, band_step, n_threads, C.byref(st), None)
     return rho, int(n), st.as_dict()
+
+
+def volume(pd, tetra, grid_dim, box_len, center, supersampling, seed=0, p_mass=None,
+           grid=None, n_threads=0):
+    """Oracle volumetric grid (grid_dim^3, [x][y][z]): compute_3d_density when
+    p_mass is given (pd[:, 3] <- rho), else interpolate_to_3d_grid of the
+    field in pd[:, 3].  Returns (grid, stats)."""
+    if grid is None:
+        grid = np.zeros(grid_dim ** 3)
+    st = Stats()
+    args = (grid_dim, box_len, grid)
+    if p_mass is None:
+        lib().sdo_interpolate_3d(pd, pd.shape[0], tetra, tetra.shape[0], *args, center[0],
+                                 center[1], center[2], supersampling, seed, n_threads,
+                                 C.byref(st))
+    else:
+        lib().sdo_compute_3d_density(pd, pd.shape[0], tetra, tetra.shape[0], *args, p_mass,
+                                     center[0], center[1], center[2], supersampling, seed,
+                                     n_threads, C.byref(st))
+    return grid, st.as_dict()
+
+
+def volume_columns(pd, tetra, nabla, tri, grid_dim, box_len, center, supersampling, seed=0,
+                   entries=None, grid=None, n_threads=0):
+    """The column loop alone, given the field, gradients and triangles;
+    entries (optional int32, one per ray in (column, is, js) order) replace
+    the entry walk."""
+    if grid is None:
+        grid = np.zeros(grid_dim ** 3)
+    st = Stats()
+    e = None
+    if entries is not None:
+        entries = np.ascontiguousarray(entries, dtype=np.int32)
+        e = entries.ctypes.data
+    lib().sdo_volume_columns(pd, tetra, nabla, tri, grid_dim, box_len, grid, center[0],
+                             center[1], center[2], supersampling, seed, e, n_threads,
+                             C.byref(st))
+    return grid, st.as_dict()
+
+
+def volume_ray_xy(grid_dim, box_len, center, supersampling):
+    """Ray positions of the volumetric path in (column, is, js) order
+    (src/dtfe.c:688-713), computed with the reference's operation order."""
+    G, ss = grid_dim, supersampling
+    dxy = box_len / G
+    off_x = -(box_len / 2.0) + center[0]
+    off_y = -(box_len / 2.0) + center[1]
+    ds = dxy / ss
+    i = np.repeat(np.arange(G), G * ss * ss).astype(np.float64)
+    j = np.tile(np.repeat(np.arange(G), ss * ss), G).astype(np.float64)
+    is_ = np.tile(np.repeat(np.arange(ss), ss), G * G).astype(np.float64)
+    js = np.tile(np.arange(ss), G * G * ss).astype(np.float64)
+    x = i * dxy + off_x + is_ * ds + ds / 2.
+    y = j * dxy + off_y + js * ds + ds / 2.
+    return x, y

@@ -38,6 +38,7 @@ static int *g_visit; /* pixel index of the q-th pixel the reference visits */
 static int64_t g_npix;
 static uint64_t g_pixel;
 static uint32_t g_sample, g_draw = 2;
+static int64_t g_rand_calls;
 
 static void refh_init_range(void) {
   if (g_cd_hi) return;
@@ -77,12 +78,17 @@ void refh_set_ray(uint64_t seed, uint64_t pixel, uint32_t sample) {
 }
 
 int64_t refh_sample_draws(void) { return g_sample_draws; }
+/* rand() calls since the last reset (a volumetric run that makes none took
+ * no perturbation, so its result does not depend on the draws). */
+int64_t refh_rand_calls(void) { return g_rand_calls; }
+void refh_reset_rand_calls(void) { g_rand_calls = 0; }
 uintptr_t refh_cd_size(void) {
   refh_init_range();
   return g_cd_hi - g_cd_lo;
 }
 
 int rand(void) {
+  ++g_rand_calls;
   uintptr_t ra = (uintptr_t)__builtin_return_address(0);
   if (ra >= g_cd_lo && ra < g_cd_hi && g_visit) {
     int64_t q = g_sample_draws++;

@@ -31,6 +31,7 @@ UNITS = {
     "walk_fp64.hip": ["-ffp-contract=off"],
     "walk_mixed.hip": ["-ffp-contract=off"],
     "batch.hip": ["-ffp-contract=off"],
+    "volume.hip": ["-ffp-contract=off"],
     "capi.cpp": ["-x", "hip", "-ffp-contract=off"],
     "server.cpp": ["-x", "hip", "-ffp-contract=off"],
 }

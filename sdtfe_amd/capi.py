@@ -63,6 +63,11 @@ class ServerInfo(C.Structure):
                 ("launches", C.c_int64), ("walk_ms", C.c_double)]
 
 
+class VolumeParams(C.Structure):
+    _fields_ = [("grid_dim", C.c_int), ("supersampling", C.c_uint), ("box_len", C.c_double),
+                ("center", C.c_double * 3), ("seed", C.c_uint64), ("max_steps", C.c_int)]
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
@@ -88,6 +93,8 @@ _SIGS = {
     "sdtfe_map": (C.c_int, [_vp, C.POINTER(MapParams), _dp, _vp, C.POINTER(Stats)]),
     "sdtfe_map_device": (C.c_int, [_vp, C.POINTER(MapParams), _vp, C.POINTER(Stats)]),
     "sdtfe_tess_sync": (C.c_int, [_vp]),
+    "sdtfe_volume": (C.c_int, [_vp, C.POINTER(VolumeParams), _dp, C.POINTER(Stats)]),
+    "sdtfe_volume_device": (C.c_int, [_vp, C.POINTER(VolumeParams), _vp, C.POINTER(Stats)]),
     "sdtfe_batch_create": (C.c_int, [C.POINTER(_vp)]),
     "sdtfe_batch_destroy": (None, [_vp]),
     "sdtfe_batch_run": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(MapParams),
@@ -110,6 +117,10 @@ _SIGS = {
                                C.c_double]),
     "pre_compute_vol": (None, [C.POINTER(_vp), C.c_int, _ip, C.c_int, C.c_float]),
     "gradients": (_vp, [_ip, C.c_int, _dp]),
+    "interpolate_to_3d_grid": (None, [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double, _dp,
+                                      C.c_double, C.c_double, C.c_double, C.c_uint]),
+    "compute_3d_density": (None, [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double, _dp,
+                                  C.c_float, C.c_double, C.c_double, C.c_double, C.c_uint]),
 }
 
 _lib = None
@@ -298,6 +309,35 @@ class Tessellation:
                                      C.byref(st) if st is not None else None))
         return st.as_dict() if st is not None else None
 
+    @staticmethod
+    def volume_params(grid_dim, supersampling=1, box_len=1.0, center=(0.5, 0.5, 0.5), seed=0,
+                      max_steps=0) -> VolumeParams:
+        p = VolumeParams()
+        p.grid_dim, p.supersampling = int(grid_dim), int(supersampling)
+        p.box_len = float(box_len)
+        p.center[:] = [float(c) for c in center]
+        p.seed, p.max_steps = int(seed), int(max_steps)
+        return p
+
+    def volume(self, grid_dim, supersampling=1, grid=None, **kw):
+        """The volumetric grid of the vertex field on the GPU
+        (interpolate_to_3d_grid); returns (grid, stats), grid flat
+        grid_dim^3 in [x][y][z] order, accumulated into if given."""
+        p = self.volume_params(grid_dim, supersampling, **kw)
+        cells = grid_dim ** 3
+        out = np.zeros(cells) if grid is None else grid
+        if out.dtype != np.float64 or not out.flags.c_contiguous or out.size != cells:
+            raise ValueError("grid must be a contiguous float64 array of grid_dim^3")
+        st = Stats()
+        check(lib().sdtfe_volume(self.h, C.byref(p), out, C.byref(st)))
+        return out, st.as_dict()
+
+    def volume_device(self, params: VolumeParams, grid_dev: "DeviceBuffer", stats: bool = True):
+        st = Stats() if stats else None
+        check(lib().sdtfe_volume_device(self.h, C.byref(params), grid_dev.ptr,
+                                        C.byref(st) if st is not None else None))
+        return st.as_dict() if st is not None else None
+
     def sync(self):
         check(lib().sdtfe_tess_sync(self.h))
 
@@ -409,6 +449,30 @@ def compute_density(particle_data, tetra_data, grid_dim, box_len, box_len_z, rho
     lib().compute_density(particle_data, particle_data.shape[0], tet, tet.shape[0], grid_dim,
                           box_len, box_len_z, rho, p_mass, center[0], center[1], center[2],
                           n_mc_samp, delta_sample)
+
+
+def interpolate_to_3d_grid(particle_data, tetra_data, grid_dim, box_len, grid, center,
+                           supersampling) -> None:
+    """The drop-in C entry point (src/dtfe.h:17-18) through the ABI: the
+    field in particle_data[:, 3] is sampled into grid (accumulated)."""
+    if particle_data.dtype != np.float64 or not particle_data.flags.c_contiguous:
+        raise ValueError("particle_data must be C-contiguous float64")
+    tet = _c64(tetra_data, np.int32)
+    lib().interpolate_to_3d_grid(particle_data, particle_data.shape[0], tet, tet.shape[0],
+                                 grid_dim, box_len, grid, center[0], center[1], center[2],
+                                 supersampling)
+
+
+def compute_3d_density(particle_data, tetra_data, grid_dim, box_len, rho, p_mass, center,
+                       supersampling) -> None:
+    """The drop-in C entry point (src/dtfe.h:20-21) through the ABI:
+    particle_data[:, 3] is overwritten and rho accumulated in place."""
+    if particle_data.dtype != np.float64 or not particle_data.flags.c_contiguous:
+        raise ValueError("particle_data must be C-contiguous float64")
+    tet = _c64(tetra_data, np.int32)
+    lib().compute_3d_density(particle_data, particle_data.shape[0], tet, tet.shape[0],
+                             grid_dim, box_len, rho, p_mass, center[0], center[1], center[2],
+                             supersampling)
 
 
 def set_seed(seed: int) -> None:

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "sdtfe_host.h"
+#include "walk_fp64_core.h"
 
 namespace sdtfe {
 
@@ -36,6 +37,7 @@ void launch_trace_mixed(sdtfe_tess *t, const MapGeom &g, int n, const double *x,
                         const double *y, const int *entry, const uint64_t *pixel,
                         const uint32_t *sample, double *value, int *crossed);
 void launch_locate(sdtfe_tess *t, int n, const double *x, const double *y, int *entry);
+void launch_volume_fp64(sdtfe_tess *t, const VolGeom &v, double *grid);
 
 extern const bool kHaveMixed;
 
@@ -99,6 +101,57 @@ int check_params(const sdtfe_map_params *p) {
     return fail(SDTFE_EINVAL, "unknown precision");
   if (p->precision == SDTFE_PREC_MIXED && !kHaveMixed)
     return fail(SDTFE_EINVAL, "mixed-precision walk not built");
+  return 0;
+}
+
+int check_volume_params(const sdtfe_volume_params *p) {
+  if (!p) return fail(SDTFE_EINVAL, "null volume parameters");
+  if (p->grid_dim <= 0 || p->grid_dim > 65536) return fail(SDTFE_EINVAL, "grid_dim out of range");
+  if (p->supersampling == 0 || p->supersampling > 1024)
+    return fail(SDTFE_EINVAL, "supersampling must be 1 .. 1024");  // src/dtfe.c:669
+  if ((int64_t)p->grid_dim * (int64_t)p->supersampling > INT_MAX)
+    return fail(SDTFE_EINVAL, "grid_dim * supersampling too large");
+  if (!(p->box_len > 0)) return fail(SDTFE_EINVAL, "box_len must be > 0");
+  return 0;
+}
+
+// src/dtfe.c:386-409, 686-695
+VolGeom make_vol_geom(const sdtfe_volume_params &p) {
+  VolGeom v{};
+  const unsigned ss = p.supersampling;
+  v.dxy = p.box_len / p.grid_dim;
+  v.ds = v.dxy / ss;
+  v.off_x = -(p.box_len / 2.0) + p.center[0];
+  v.off_y = -(p.box_len / 2.0) + p.center[1];
+  v.off_z = -(p.box_len / 2.0) + p.center[2];
+  v.lz = p.box_len;
+  const double dz = v.lz / p.grid_dim;
+  v.dzs = dz / ss;
+  v.ss_weight = 1. / (ss * ss * ss);
+  v.pert_step = (v.dxy / 2.0) / kPerturbMax;
+  v.seed = p.seed;
+  v.grid = p.grid_dim;
+  v.ss = (int)ss;
+  v.nzs = p.grid_dim * (int)ss;
+  v.max_steps = p.max_steps > 0 ? p.max_steps : SDTFE_MAX_STEPS_DEFAULT;
+  return v;
+}
+
+int run_volume(sdtfe_tess *t, const sdtfe_volume_params *p, double *grid, double *t_walk_ms) {
+  const VolGeom v = make_vol_geom(*p);
+  SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
+  SDTFE_HIP(hipEventRecord(t->ev0, t->stream));
+  SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
+  launch_volume_fp64(t, v, grid);
+  SDTFE_HIP(hipGetLastError());
+  SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
+  SDTFE_HIP(hipEventRecord(t->ev1, t->stream));
+  if (t_walk_ms) {
+    SDTFE_HIP(hipEventSynchronize(t->ev1));
+    float ms = 0;
+    SDTFE_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    *t_walk_ms = ms;
+  }
   return 0;
 }
 
@@ -485,6 +538,58 @@ int sdtfe_map_device(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_devic
   return 0;
 }
 
+int sdtfe_volume(sdtfe_tess *t, const sdtfe_volume_params *p, double *grid,
+                 sdtfe_stats *stats) {
+  if (!t) return fail(SDTFE_EINVAL, "null tessellation");
+  int rc = check_volume_params(p);
+  if (rc) return rc;
+  if (!grid) return fail(SDTFE_EINVAL, "null grid");
+  auto t0 = Clock::now();
+  const size_t cells = (size_t)p->grid_dim * (size_t)p->grid_dim * (size_t)p->grid_dim;
+  double *d = nullptr;
+  if (dev_malloc((void **)&d, sizeof(double) * cells, t->stream) != hipSuccess)
+    return fail(SDTFE_ENOMEM, "volumetric grid allocation failed");
+  // the walk adds into the caller's values in the reference's order, so
+  // they go up first
+  double t_walk = 0;
+  rc = hip_check(hipMemcpyAsync(d, grid, sizeof(double) * cells, hipMemcpyHostToDevice,
+                                t->stream),
+                 "grid upload");
+  if (!rc) rc = run_volume(t, p, d, &t_walk);
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(grid, d, sizeof(double) * cells, hipMemcpyDeviceToHost,
+                                  t->stream),
+                   "grid download");
+  dev_free(d, t->stream);
+  if (!rc) rc = hip_check(hipStreamSynchronize(t->stream), "sync");
+  if (rc) return rc;
+  if ((rc = read_stats(t, stats))) return rc;
+  if (stats) {
+    stats->t_walk_ms = t_walk;
+    stats->t_kernel_ms = t_walk;
+    stats->t_total_ms = ms_since(t0);
+  }
+  return 0;
+}
+
+int sdtfe_volume_device(sdtfe_tess *t, const sdtfe_volume_params *p, double *grid_device,
+                        sdtfe_stats *stats) {
+  if (!t) return fail(SDTFE_EINVAL, "null tessellation");
+  int rc = check_volume_params(p);
+  if (rc) return rc;
+  if (!grid_device) return fail(SDTFE_EINVAL, "null grid");
+  auto t0 = Clock::now();
+  double t_walk = 0;
+  if ((rc = run_volume(t, p, grid_device, stats ? &t_walk : nullptr))) return rc;
+  if (stats) {
+    if ((rc = read_stats(t, stats))) return rc;
+    stats->t_walk_ms = t_walk;
+    stats->t_kernel_ms = t_walk;
+    stats->t_total_ms = ms_since(t0);
+  }
+  return 0;
+}
+
 int sdtfe_tess_sync(sdtfe_tess *t) {
   if (!t) return fail(SDTFE_EINVAL, "null tessellation");
   SDTFE_HIP(hipStreamSynchronize(t->stream));
@@ -570,6 +675,51 @@ void pre_compute_vol(double **particle_data, int n_particles, int *tetra_data, i
       for (int i = 0; i < n_particles; ++i) (*particle_data)[4 * (size_t)i + 3] = r[i];
   }
   if (rc) std::fprintf(stderr, "pre_compute_vol (sdtfe_amd): %s\n", g_err.c_str());
+  sdtfe_tess_destroy(t);
+}
+
+static int dropin_volume(sdtfe_tess *t, int grid_dim, double box_len, double *grid,
+                         double center_x, double center_y, double center_z,
+                         unsigned supersampling) {
+  sdtfe_volume_params p{};
+  p.grid_dim = grid_dim;
+  p.supersampling = supersampling;
+  p.box_len = box_len;
+  p.center[0] = center_x;
+  p.center[1] = center_y;
+  p.center[2] = center_z;
+  p.seed = g_seed.fetch_add(1);
+  return sdtfe_volume(t, &p, grid, nullptr);
+}
+
+void interpolate_to_3d_grid(double *particle_data, int n_particles, int *tetra_data,
+                            int n_tetra, int grid_dim, double box_len, double *grid,
+                            double center_x, double center_y, double center_z,
+                            unsigned supersampling) {
+  sdtfe_tess *t = nullptr;
+  int rc = sdtfe_tess_create_field(particle_data, n_particles, tetra_data, n_tetra, &t);
+  if (rc == 0)
+    rc = dropin_volume(t, grid_dim, box_len, grid, center_x, center_y, center_z,
+                       supersampling);
+  if (rc) std::fprintf(stderr, "interpolate_to_3d_grid (sdtfe_amd): %s\n", g_err.c_str());
+  sdtfe_tess_destroy(t);
+}
+
+void compute_3d_density(double *particle_data, int n_particles, int *tetra_data, int n_tetra,
+                        int grid_dim, double box_len, double *rho, float p_mass,
+                        double center_x, double center_y, double center_z,
+                        unsigned supersampling) {
+  sdtfe_tess *t = nullptr;
+  int rc = sdtfe_tess_create(particle_data, n_particles, tetra_data, n_tetra, p_mass, &t);
+  if (rc == 0 && n_particles > 0) {
+    std::vector<double> r((size_t)n_particles);
+    rc = sdtfe_tess_vertex_density(t, r.data());
+    if (rc == 0)
+      for (int i = 0; i < n_particles; ++i) particle_data[4 * (size_t)i + 3] = r[i];
+  }
+  if (rc == 0)
+    rc = dropin_volume(t, grid_dim, box_len, rho, center_x, center_y, center_z, supersampling);
+  if (rc) std::fprintf(stderr, "compute_3d_density (sdtfe_amd): %s\n", g_err.c_str());
   sdtfe_tess_destroy(t);
 }
 

@@ -111,6 +111,19 @@ struct MapGeom {
   int grid, samples, max_steps;
 };
 
+// A volumetric grid (src/dtfe.c:667-735): grid^3 cells, ss x ss rays per
+// column and ss fine z points per cell.
+struct VolGeom {
+  double dxy, ds;              // cell width, ray spacing (dxy / ss)
+  double off_x, off_y, off_z;  // grid corner
+  double lz;                   // ray length (the cube's side)
+  double dzs;                  // fine z spacing (lz / grid / ss)
+  double ss_weight;            // 1 / ss^3
+  double pert_step;            // (dxy / 2) / PERTURB_MAX
+  uint64_t seed;
+  int grid, ss, nzs, max_steps;
+};
+
 enum StatIdx {
   ST_SAMPLES = 0,
   ST_MISSED,

@@ -1,4 +1,6 @@
-"""Drop-in for the reference's ``pydtfe.compute_surface_density``.
+"""Drop-in for the reference's ``pydtfe`` module: ``compute_surface_density``,
+``compute_3d_density``, ``interpolate_field2grid`` and
+``compute_particle_density`` (python/pydtfe.cpp:16-203).
 
 Same function name, argument names, defaults and errors as the pybind11
 binding (python/pydtfe.cpp:16-66, 207-261); the map itself is computed by
@@ -26,6 +28,7 @@ the unseeded libc rand().  Here both come from the counter-based stream with
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 
 import numpy as np
@@ -140,3 +143,73 @@ def compute_surface_density(particle_pos, center, grid_dim, sample_factor, box_w
                          float(np.float32(mass)), [float(c) for c in center],
                          int(mc_sample_count), float(mc_box_width))
     return rho.reshape(g, g)
+
+
+def _positions(particle_pos) -> np.ndarray:
+    pos = np.asarray(particle_pos, dtype=np.float32, order="F")
+    if pos.ndim != 2 or pos.shape[1] != 3:
+        raise ValueError("incorrect shape of particle position array")
+    return pos
+
+
+def compute_3d_density(particle_pos, center, grid_dim, sample_factor, box_length, particle_mass,
+                       supersampling=1, *, seed: int = 0):
+    """Volumetric density (grid_dim^3, float64, mass/length^3, indexed
+    [x, y, z]) of the DTFE field of ``particle_pos`` on the cube of side
+    ``box_length`` about ``center``; each cell averages supersampling^3
+    points.  Restates python/pydtfe.cpp:68-111: float32 positions, shuffle
+    of the first N * sample_factor particles, mass / sample_factor, Qhull,
+    then compute_3d_density (src/dtfe.h:20-21) of libsdtfe_amd.so."""
+    pos = _positions(particle_pos)
+    n = pos.shape[0]
+    pd = convert_to_qhdata(pos)
+    n_shuffle = int(np.float32(n) * np.float32(sample_factor))
+    if n_shuffle < 5:
+        raise ValueError("not enough particles")
+    shuffle_buff(pd, n, n_shuffle, seed)
+    mass = float(particle_mass) / float(np.float32(sample_factor))
+    pd = np.ascontiguousarray(pd[:n_shuffle])
+    tetra = triangulate(pd)
+    g = int(grid_dim)
+    rho = np.zeros(g ** 3)
+    capi.set_seed(seed)
+    capi.compute_3d_density(pd, tetra, g, float(box_length), rho, float(np.float32(mass)),
+                            [float(c) for c in center], int(supersampling))
+    return rho.reshape(g, g, g)
+
+
+def interpolate_field2grid(particle_pos, particle_value, center, grid_dim, sample_factor,
+                           box_length, supersampling=1, *, seed: int = 0):
+    """A particle field (float32 values, one per particle) sampled on the
+    grid_dim^3 grid, as compute_3d_density samples the density.  Restates
+    python/pydtfe.cpp:113-163 (the same errors, sample_factor 1 only), then
+    interpolate_to_3d_grid (src/dtfe.h:17-18) of libsdtfe_amd.so."""
+    pos = _positions(particle_pos)
+    n = pos.shape[0]
+    val = np.asarray(particle_value, dtype=np.float32)
+    if n != val.shape[0]:
+        raise ValueError("different length of particle position and value arrays")
+    pd = convert_to_qhdata(pos)
+    if np.float32(sample_factor) != np.float32(1.0):
+        raise ValueError("`sample_factor != 1.0f` not supported at the moment")
+    tetra = triangulate(pd)
+    pd[:, 3] = val.astype(np.float64)
+    g = int(grid_dim)
+    grid = np.zeros(g ** 3)
+    capi.set_seed(seed)
+    capi.interpolate_to_3d_grid(pd, tetra, g, float(box_length), grid,
+                                [float(c) for c in center], int(supersampling))
+    return grid.reshape(g, g, g)
+
+
+def compute_particle_density(particle_pos, particle_mass):
+    """DTFE density at each particle (float32, mass/length^3): Qhull, then
+    pre_compute_vol (src/triangulate.h:26) of libsdtfe_amd.so; restates
+    python/pydtfe.cpp:165-191."""
+    pos = _positions(particle_pos)
+    pd = convert_to_qhdata(pos)
+    tetra = triangulate(pd)
+    ptr = C.c_void_p(pd.ctypes.data)
+    capi.lib().pre_compute_vol(C.byref(ptr), pd.shape[0], tetra, tetra.shape[0],
+                               float(np.float32(particle_mass)))
+    return pd[:, 3].astype(np.float32)

@@ -7,7 +7,8 @@ made from /root/reference by ``make -C oracle ref``):
 
 Every output below comes from the reference's own functions
 (pre_compute_vol, gradients, find_forward_hull, copy_fh_triangles, pt_loc_2d,
-shoot_3d_ray, compute_density) with libc rand() answered from the shared
+shoot_3d_ray, compute_density, compute_3d_density, interpolate_to_3d_grid,
+evaluate_3d_ray_on_grid) with libc rand() answered from the shared
 sample stream (oracle/ref_harness.c).  Inputs are stored beside the outputs;
 the one large tessellation (the 32,768-particle map) is stored as a SHA-256
 of its bytes and rebuilt by the tests.
@@ -206,6 +207,93 @@ def g5_utils(R):
                         theta=theta, center=center, rotated=rot[:, :3])
 
 
+VOLUMES = [
+    # name, grid, supersampling, box_len, center, p_mass (None: given field), seed
+    ("dens", 16, 2, 1.0, (0.5, 0.5, 0.5), 0.37, 0),
+    ("field", 12, 3, 1.3, (0.45, 0.55, 0.5), None, 0),
+]
+
+
+def _field(pos):
+    """A vertex field that is not linear (so gradients vary per tet)."""
+    return 1.0 + 2.0 * pos[:, 0] - 3.0 * pos[:, 1] + 0.5 * pos[:, 2] + np.sin(7.0 * pos[:, 2])
+
+
+def _volume_by_rays(R, pd, tetra, G, ss, L, cen, seed, grid):
+    """interpolate_to_3d_grid's column loop (src/dtfe.c:686-731) driven from
+    here so that each ray's perturbation draws are keyed (seed, column, ray):
+    the reference's own pt_loc_2d and evaluate_3d_ray_on_grid per ray.
+    Returns the entry tets."""
+    nt = tetra.shape[0]
+    nabla_ptr = R.gradients(tetra, nt, pd)
+    bm = R.create_bitmap(nt * 4)
+    n_tri = R.find_forward_hull(tetra, nt, pd, pd.shape[0], bm)
+    tri_ptr = R.copy_fh_triangles(tetra, nt, pd, pd.shape[0], n_tri, bm)
+    R.free(bm)
+    x, y = O.volume_ray_xy(G, L, cen, ss)
+    off_z = -(L / 2.0) + cen[2]
+    dxy = L / G
+    entries = np.empty(x.size, dtype=np.int32)
+    tmp = C.c_int(0)
+    for q in range(x.size):
+        c, r = divmod(q, ss * ss)
+        start = C.c_int(0)
+        entries[q] = R.pt_loc_2d(tri_ptr, pd, C.byref(start), float(x[q]), float(y[q]))
+        R.refh_set_ray(seed, c, r)
+        R.evaluate_3d_ray_on_grid(int(entries[q]), tetra, nabla_ptr, pd, float(x[q]),
+                                  float(y[q]), off_z, L, G, dxy, C.byref(tmp), ss,
+                                  grid.ctypes.data + 8 * c * G)
+    R.free(C.c_void_p(nabla_ptr))
+    R.free(C.c_void_p(tri_ptr))
+    return entries
+
+
+def g6_volume(R):
+    """Volumetric grids (src/dtfe.c:376-516, 657-735) from the reference."""
+    rng = np.random.default_rng(1)
+    pos = rng.random((2048, 3))
+    tetra = triangulate(pos)
+    out = {"pos": pos, "tetra": tetra}
+    for name, G, ss, L, cen, m, seed in VOLUMES:
+        pd = to_particle_data(pos)
+        init = np.zeros(G ** 3) if m is not None else rng.random(G ** 3)
+        grid = init.copy()
+        R.refh_reset_rand_calls()
+        if m is not None:
+            R.compute_3d_density(pd, pd.shape[0], tetra, tetra.shape[0], G, L, grid, m,
+                                 cen[0], cen[1], cen[2], ss)
+            out[f"{name}_vertex_rho"] = pd[:, 3].copy()
+        else:
+            pd[:, 3] = _field(pos)
+            R.interpolate_to_3d_grid(pd, pd.shape[0], tetra, tetra.shape[0], G, L, grid,
+                                     cen[0], cen[1], cen[2], ss)
+            # the same grid from the per-ray driver (pins its loop restatement)
+            by_rays = init.copy()
+            _volume_by_rays(R, pd, tetra, G, ss, L, cen, seed, by_rays)
+            assert np.array_equal(by_rays, grid)
+        assert R.refh_rand_calls() == 0  # no perturbation: draws play no part
+        out[f"{name}_init"] = init
+        out[f"{name}_grid"] = grid
+    # rays exactly through vertices and along edges of a dyadic point set,
+    # which drives the perturbation (keyed draws through the per-ray driver)
+    p = np.unique(rng.integers(0, 64, size=(700, 3)).astype(np.float64) / 64.0, axis=0)
+    t = triangulate(p)
+    pd = to_particle_data(p)
+    R.pre_compute_vol(C.byref(C.c_void_p(pd.ctypes.data)), pd.shape[0], t, t.shape[0], 1.0)
+    for name, G, ss in (("crafted16", 16, 1), ("crafted8", 8, 2)):
+        grid = np.zeros(G ** 3)
+        R.refh_reset_rand_calls()
+        entries = _volume_by_rays(R, pd, t, G, ss, 1.0, (0.5, 0.5, 0.5), 7, grid)
+        out[f"{name}_grid"] = grid
+        out[f"{name}_entries"] = entries
+        out[f"{name}_rand_calls"] = R.refh_rand_calls()
+    out["crafted_pos"] = p
+    out["crafted_tetra"] = t
+    out["crafted_rho"] = pd[:, 3].copy()
+    np.savez_compressed(os.path.join(OUT, "g6_volume.npz"), **out)
+    return {k: int(v) for k, v in out.items() if k.endswith("_rand_calls")}
+
+
 def main():
     if not O.ref_available():
         O.build(ref=True)
@@ -216,7 +304,9 @@ def main():
     g3_maps(R)
     g4_c1(R)
     g5_utils(R)
-    print("golden vectors written; crafted rays that completed:", hit)
+    pert = g6_volume(R)
+    print("golden vectors written; crafted rays that completed:", hit,
+          "; volumetric perturbation draws:", pert)
 
 
 if __name__ == "__main__":

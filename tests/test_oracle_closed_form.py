@@ -270,3 +270,43 @@ def test_failures_are_counted(oracle):
     st = oracle.Stats()
     v, c = oracle.shoot_ray(entry, bad, nab, pd, x, y, -1.0, 3.0, 0.01, 0, 0, 0, st)
     assert c == 0 and v == 0.0 and st.no_intersection == 1
+
+
+@pytest.mark.parametrize("G,ss", [(8, 1), (6, 3)])
+def test_volume_of_linear_field_is_cell_mean(oracle, G, ss):
+    """Volumetric path (src/dtfe.c:376-516): with a linear vertex field and
+    the unit cube as hull, every fine point of every cell lies in exactly one
+    tet, so a cell holds the mean of the field over its ss^3 fine points,
+    which for a linear field is its value at the cell centre."""
+    rng = np.random.default_rng(3)
+    corners = np.array([[i, j, k] for i in (0, 1) for j in (0, 1) for k in (0, 1)], float)
+    pos = np.vstack([corners, 0.02 + 0.96 * rng.random((400, 3))])
+    tetra = triangulate(pos)
+    pd = to_particle_data(pos)
+    a, b, c, d = 0.7, 1.5, -2.25, 3.0
+    pd[:, 3] = a + b * pos[:, 0] + c * pos[:, 1] + d * pos[:, 2]
+    # a box inside the cube, shifted in x so that no ray meets the diagonals
+    # of the cube's faces (a ray on one is perturbed and samples the field
+    # slightly off its column)
+    L, cen = 0.9, (0.5123, 0.5, 0.5)
+    grid, st = oracle.volume(pd, tetra, G, L, cen, ss, n_threads=2)
+    h = (np.arange(G) + 0.5) * (L / G) - L / 2
+    X, Y, Z = np.meshgrid(h + cen[0], h + cen[1], h + cen[2], indexing="ij")  # [x][y][z]
+    want = a + b * X + c * Y + d * Z
+    np.testing.assert_allclose(grid.reshape(G, G, G), want, rtol=0, atol=1e-11)
+    assert st["missed"] == 0 and st["failed"] == 0 and st["perturb_steps"] == 0
+
+
+def test_volume_density_mass(oracle):
+    """The density grid integrates to the total mass (N m) when the box
+    covers the hull: each fine point samples the DTFE field, whose integral
+    is N m exactly (same argument as the map's mass check), up to the
+    quadrature error of the fine points."""
+    rng = np.random.default_rng(8)
+    pos = rng.random((3000, 3))
+    tetra = triangulate(pos)
+    pd = to_particle_data(pos)
+    m, G = 0.5, 24
+    grid, _ = oracle.volume(pd, tetra, G, 1.0, (0.5, 0.5, 0.5), 2, p_mass=m, n_threads=2)
+    total = grid.sum() / G ** 3
+    assert abs(total - 3000 * m) < 0.02 * 3000 * m

@@ -94,3 +94,48 @@ def test_rho_accumulates(oracle, golden):
     rho, _ = oracle.compute_density(pd, g["tetra"], 64, 1.0, 1.0, 1.0, (0.5, 0.5, 0.5), 4,
                                     0.0, 3, rho=base.copy())
     np.testing.assert_array_equal(rho, base + g["plain_rho"])
+
+
+def _vol_case(golden, name):
+    import tests.golden.make_golden as M
+    return next(v for v in M.VOLUMES if v[0] == name), M._field
+
+
+@pytest.mark.parametrize("name", ["dens", "field"])
+def test_volume_matches_reference(oracle, golden, name):
+    """compute_3d_density / interpolate_to_3d_grid (src/dtfe.c:376-516,
+    657-735): the oracle's grid equals the reference's bit for bit, the
+    accumulation into a non-zero grid included."""
+    g = golden("g6_volume.npz")
+    (_, G, ss, L, cen, m, seed), field = _vol_case(golden, name)
+    pd = to_particle_data(g["pos"])
+    if m is None:
+        pd[:, 3] = field(g["pos"])
+    grid, st = oracle.volume(pd, g["tetra"], G, L, cen, ss, seed=seed, p_mass=m,
+                             grid=g[f"{name}_init"].copy(), n_threads=2)
+    if m is not None:
+        np.testing.assert_array_equal(pd[:, 3], g[f"{name}_vertex_rho"])
+    np.testing.assert_array_equal(grid, g[f"{name}_grid"])
+    assert st["samples"] == G * G * ss * ss
+    assert st["perturb_steps"] == 0 and st["tet_steps"] > 0
+
+
+@pytest.mark.parametrize("name,G,ss", [("crafted16", 16, 1), ("crafted8", 8, 2)])
+def test_volume_perturbed_rays_match_reference(oracle, golden, name, G, ss):
+    """Rays through vertices and along edges of a dyadic point set: the
+    perturbation in the volumetric walk, with the reference's own entry
+    walk's tets and keyed draws."""
+    g = golden("g6_volume.npz")
+    pd = to_particle_data(g["crafted_pos"])
+    pd[:, 3] = g["crafted_rho"]
+    tetra = g["crafted_tetra"]
+    nabla = oracle.gradients(tetra, pd)
+    tri = oracle.triangles(tetra, pd)
+    x, y = oracle.volume_ray_xy(G, 1.0, (0.5, 0.5, 0.5), ss)
+    entries = np.array([oracle.pt_loc_2d(tri, pd, float(a), float(b)) for a, b in zip(x, y)],
+                       dtype=np.int32)
+    np.testing.assert_array_equal(entries, g[f"{name}_entries"])
+    grid, st = oracle.volume_columns(pd, tetra, nabla, tri, G, 1.0, (0.5, 0.5, 0.5), ss,
+                                     seed=7, n_threads=2)
+    np.testing.assert_array_equal(grid, g[f"{name}_grid"])
+    assert st["perturb_steps"] == int(g[f"{name}_rand_calls"]) > 100
