@@ -1,6 +1,7 @@
 """Surface-density map throughput on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config clustered|uniform|lattice]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--config clustered|uniform|lattice|cutouts|serve|volume]
 
 A step is one map of the workload with the tessellation resident in HBM:
 Monte-Carlo sample-rays are placed, located on the forward hull and walked
@@ -284,13 +285,108 @@ def bench_serve(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_volume(a):
+    """Volumetric grid (compute_3d_density's interpolation, src/dtfe.c:
+    667-735) of the 1,048,576 clustered particles' density: a 512^3 grid of
+    the unit cube, 2 x 2 rays per column and 2 points per cell along each
+    (1,048,576 rays, 268M field samples), FP64, accumulated in HBM.  The
+    oracle computes the whole grid on the host cores, which is both the CPU
+    baseline and a bit-for-bit check of the GPU grid."""
+    from sdtfe_amd import capi
+    from sdtfe_amd.tessellate import to_particle_data
+    G, ss = a.vol_grid, a.vol_ss
+    pos = particles("clustered")
+    t0 = time.perf_counter()
+    tetra = tessellation("clustered", pos, a.tess_cache)
+    log(f"[bench] tessellated {pos.shape[0]} particles -> {tetra.shape[0]} tets in "
+        f"{time.perf_counter() - t0:.1f} s")
+    tess = capi.Tessellation(to_particle_data(pos), tetra)
+    cells = G ** 3
+    buf = capi.DeviceBuffer(cells * 8)
+    p = tess.volume_params(G, ss, seed=a.seed)
+    buf.zero()
+    for _ in range(a.warmup):
+        tess.volume_device(p, buf, stats=True)
+    walk, st = [], None
+    tess.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        st = tess.volume_device(p, buf, stats=True)
+        walk.append(st["t_walk_ms"])
+    tess.sync()
+    wall = time.perf_counter() - t0
+    # one grid, untimed, into a clean buffer for the comparison
+    buf.zero()
+    tess.volume_device(p, buf, stats=False)
+    tess.sync()
+    grid = buf.to_host()
+    buf.close()
+    tess.close()
+    rays = G * G * ss * ss
+    ms = 1e3 * wall / a.steps
+    kernel_ms = float(np.mean(walk))
+    log(f"[bench] volume {G}^3 x{ss}: {ms:.2f} ms/grid (kernel {kernel_ms:.2f} ms), "
+        f"{st['tet_steps'] / rays:.1f} tets/ray")
+    # algorithmic bytes: 88 per tet crossed, plus one read and one write of
+    # every cell each ray passes (the cells of its column inside the hull)
+    cell_visits = rays * G
+    alg = st["tet_steps"] * BYTES_PER_STEP + 16 * cell_visits
+    achieved = alg / (kernel_ms / 1e3) / 1e9
+    workload = (f"clustered-{pos.shape[0]} {G}^3 volumetric grid, {ss}x{ss} rays per column, "
+                f"{ss} points per cell in z")
+    tr = measured_traffic("volume_fp64_kernel", workload)
+    cpu = None
+    if not a.no_cpu:
+        from oracle import oracle as O
+        if not os.path.exists(O.LIB_PATH):
+            O.build()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        pd = to_particle_data(pos)
+        log(f"[bench] CPU baseline: the whole grid on {threads} threads")
+        t0 = time.perf_counter()
+        want, st_o = O.volume(pd, tetra, G, 1.0, (0.5, 0.5, 0.5), ss, seed=a.seed, p_mass=1.0,
+                              n_threads=threads)
+        t_cpu = time.perf_counter() - t0
+        same = bool(np.array_equal(grid, want))
+        log(f"[bench] CPU {rays / t_cpu:.3e} rays/s ({t_cpu:.1f} s incl. density and "
+            f"gradients); GPU grid bit-equal: {same}")
+        cpu = {"value": rays / t_cpu, "unit": "rays/s", "cores": threads, "kind": "port",
+               "sample": f"the whole {G}^3 grid (density, gradients, entry walks and rays), "
+                         f"{t_cpu:.1f} s",
+               "grid_bit_equal": same}
+    out = {"metric": "rays/s", "value": round(rays / (ms / 1e3), 1), "unit": "rays/s",
+           "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": workload, "grid": G, "supersampling": ss, "rays": rays,
+                      "cells": cells},
+           "cells_per_s": round(cells / (ms / 1e3), 1),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": None if tr is None else tr["bytes_per_launch"],
+                        "kernel": "volume_fp64_kernel",
+                        "bytes_per_step": BYTES_PER_STEP, "bytes_per_cell_visit": 16,
+                        "tet_steps_per_launch": st["tet_steps"], "cell_visits": cell_visits,
+                        "kernel_ms": round(kernel_ms, 3)},
+           "cpu_baseline": cpu,
+           "stats": {k: st[k] for k in ("samples", "missed", "failed", "tet_steps",
+                                        "perturb_steps")}}
+    if tr is not None:
+        out["roofline"]["physical_gbs"] = round(tr["bytes_per_launch"] / (kernel_ms / 1e3) / 1e9, 1)
+    if cpu is not None:
+        out["gpu_over_cpu"] = round(out["value"] / cpu["value"], 2)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="clustered",
-                    choices=sorted(CONFIGS) + ["cutouts", "serve"])
+                    choices=sorted(CONFIGS) + ["cutouts", "serve", "volume"])
+    ap.add_argument("--vol-grid", type=int, default=512, help="volume: cells per side")
+    ap.add_argument("--vol-ss", type=int, default=2, help="volume: supersampling")
     ap.add_argument("--cutouts", type=int, default=512, help="cutouts / requests (cutouts, serve)")
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -309,8 +405,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    if a.config in ("cutouts", "serve"):
-        (bench_cutouts if a.config == "cutouts" else bench_serve)(a)
+    if a.config in ("cutouts", "serve", "volume"):
+        {"cutouts": bench_cutouts, "serve": bench_serve, "volume": bench_volume}[a.config](a)
         return
 
     from sdtfe_amd import capi

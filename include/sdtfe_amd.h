@@ -263,7 +263,8 @@ int sdtfe_server_wait(sdtfe_server *s, int64_t request_id, sdtfe_request_info *i
 int sdtfe_server_get_info(sdtfe_server *s, sdtfe_server_info *info);
 void sdtfe_server_destroy(sdtfe_server *s);
 
-/* Device buffers for callers without another HIP allocator. */
+/* Device buffers for callers without another HIP allocator.  The copies
+ * and the fill return when they are complete. */
 void *sdtfe_device_alloc(size_t bytes);
 int sdtfe_device_free(void *p);
 int sdtfe_memcpy_d2h(void *dst, const void *src, size_t bytes);

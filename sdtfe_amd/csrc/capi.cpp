@@ -614,10 +614,15 @@ int sdtfe_memcpy_d2h(void *dst, const void *src, size_t bytes) {
 }
 int sdtfe_memcpy_h2d(void *dst, const void *src, size_t bytes) {
   SDTFE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  // from pageable memory the call may return before the device copy lands
+  SDTFE_HIP(hipStreamSynchronize(nullptr));
   return 0;
 }
 int sdtfe_memset_d(void *p, int value, size_t bytes) {
-  SDTFE_HIP(hipMemset(p, value, bytes));
+  // waits for the fill: tessellation streams do not order against the
+  // null stream, so a later launch on one must not overtake it
+  SDTFE_HIP(hipMemsetAsync(p, value, bytes, nullptr));
+  SDTFE_HIP(hipStreamSynchronize(nullptr));
   return 0;
 }
 int sdtfe_device_count(void) {

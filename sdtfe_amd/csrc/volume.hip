@@ -25,20 +25,66 @@ SDTFE_HD inline void fine_range(const VolGeom &v, double z_in, double z_out, int
   l = cl > (double)v.nzs ? v.nzs : (cl < 0.0 ? 0 : (int)cl);
 }
 
+// Sixteen consecutive cells of a column (one 128-byte line of the grid,
+// base a multiple of 16) staged in this lane's slot of LDS: a visit reads
+// and writes whole lines, and the cells a ray passes are then added in LDS.
+// (With thousands of columns in flight the lines do not stay in L2 between
+// a lane's consecutive cells, so cell-by-cell access would fetch each line
+// again for every cell.)  Cells at or beyond n are neither read nor
+// written.  pairs: every column starts 16-byte aligned (an even grid size
+// and an aligned grid), so lines move as 16-byte pieces.
+constexpr int kWinCells = 16;
+constexpr int kWinStride = 18;  // doubles per lane slot (16-byte aligned, fewer bank conflicts)
+
+struct CellWindow {
+  double *w;
+  int base = -1;
+
+  SDTFE_HD void load(const double *col, int b, int n, bool pairs) {
+    base = b;
+    if (b + kWinCells <= n) {
+      if (pairs) {
+        const double2 *src = reinterpret_cast<const double2 *>(col + b);
+        double2 *dst = reinterpret_cast<double2 *>(w);
+#pragma unroll
+        for (int q = 0; q < kWinCells / 2; ++q) dst[q] = src[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < kWinCells; ++q) w[q] = col[b + q];
+      }
+    } else {
+      for (int q = 0; q < kWinCells; ++q) w[q] = b + q < n ? col[b + q] : 0.0;
+    }
+  }
+  SDTFE_HD void store(double *col, int n, bool pairs) const {
+    if (base + kWinCells <= n) {
+      if (pairs) {
+        double2 *dst = reinterpret_cast<double2 *>(col + base);
+        const double2 *src = reinterpret_cast<const double2 *>(w);
+#pragma unroll
+        for (int q = 0; q < kWinCells / 2; ++q) dst[q] = src[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < kWinCells; ++q) col[base + q] = w[q];
+      }
+    } else {
+      for (int q = 0; q < kWinCells && base + q < n; ++q) col[base + q] = w[q];
+    }
+  }
+};
+
 // evaluate_3d_ray_on_grid (src/dtfe.c:376-516): the walk of shoot_3d_ray
-// with its perturbation; col is the ray's column.  The column's current
-// cell is held in a register (c_iz, c_val) and written back when the walk
-// moves to another cell, so each cell is read and written once per visit.
-// A failed walk keeps what it added, as the reference does.  Returns the
-// tets crossed (0 on failure).
+// with its perturbation; col is the ray's column, whose cells are read and
+// written a line at a time through win (CellWindow).  Every term is added to
+// its cell in the reference's order.  A failed walk keeps what it added, as the
+// reference does.  Returns the tets crossed (0 on failure).
 SDTFE_HD inline int column_walk64(const Rec64 *__restrict__ rec, int entry, double x, double y,
                                   const VolGeom &v, uint64_t key, double *col,
-                                  LaneCounters &cnt) {
+                                  CellWindow &win, bool pairs, LaneCounters &cnt) {
   double ox = x, oy = y;
   uint32_t draw = 2;
   int cur = entry, prev = -1, f_in = -1, f_out = 0, nxt = entry, steps = 0, crossed = 0;
-  int c_iz = -1;
-  double c_val = 0.0;
+  win.base = -1;
   bool first = true;
   Tet64 r;
   for (;;) {
@@ -80,22 +126,26 @@ SDTFE_HD inline int column_walk64(const Rec64 *__restrict__ rec, int entry, doub
       for (int k = e; k < l; ++k) {
         const int iz = k / v.ss;
         const double z = v.off_z + v.dzs * (double)k + v.dzs / 2.;
-        if (iz != c_iz) {
-          if (c_iz >= 0) col[c_iz] = c_val;
-          c_iz = iz;
-          c_val = col[iz];
+        if ((iz & ~(kWinCells - 1)) != win.base) {
+          if (win.base >= 0) win.store(col, v.grid, pairs);
+          win.load(col, iz & ~(kWinCells - 1), v.grid, pairs);
         }
-        c_val = c_val + (fixed + r.gz * z) * v.ss_weight;
+        double &cell = win.w[iz & (kWinCells - 1)];
+        cell = cell + (fixed + r.gz * z) * v.ss_weight;
       }
     }
   }
-  if (c_iz >= 0) col[c_iz] = c_val;
+  if (win.base >= 0) win.store(col, v.grid, pairs);
   return crossed;
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void
 volume_fp64_kernel(const Rec64 *__restrict__ rec, EntryGrid eg, VolGeom v, double *grid,
                    unsigned long long *stats) {
+  __shared__ __attribute__((aligned(16))) double lds[256 * kWinStride];
+  CellWindow win;
+  win.w = lds + threadIdx.x * kWinStride;
+  const bool pairs = (v.grid & 1) == 0 && ((uintptr_t)grid & 15) == 0;
   // 16 x 16 columns per block, an 8 x 8 patch per wave
   const int tiles = (v.grid + 15) / 16;
   const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
@@ -118,7 +168,7 @@ volume_fp64_kernel(const Rec64 *__restrict__ rec, EntryGrid eg, VolGeom v, doubl
           continue;
         }
         const uint64_t key = sample_key(v.seed, c, (uint32_t)(is * v.ss + js));
-        const int crossed = column_walk64(rec, entry, x, y, v, key, col, cnt);
+        const int crossed = column_walk64(rec, entry, x, y, v, key, col, win, pairs, cnt);
         cnt.tet_steps += (unsigned)crossed;
         if (crossed == 0) cnt.failed++;
       }
