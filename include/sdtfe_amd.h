@@ -149,8 +149,9 @@ int sdtfe_tess_create_field(const double *particle_data, int n_particles,
 void sdtfe_tess_destroy(sdtfe_tess *t);
 int sdtfe_tess_get_info(const sdtfe_tess *t, sdtfe_tess_info *info);
 /* Copies out: vertex densities (N), gradients (T x 3), forward-hull
- * triangles as (a, b, c, -1, -1, -1, tet, 0) rows in the reference's order
- * (neighbour slots are not built: the grid lookup does not walk them). */
+ * triangles as copy_fh_triangles (src/triangulate.c:318-395) writes them:
+ * (a, b, c, n_CB, n_AC, n_BA, tet, 0) rows in the reference's order, the
+ * neighbour across each edge or -1 (slot 7, unwritten there, is 0). */
 int sdtfe_tess_vertex_density(const sdtfe_tess *t, double *rho);
 int sdtfe_tess_gradients(const sdtfe_tess *t, double *nabla);
 int sdtfe_tess_triangles(const sdtfe_tess *t, int *tri_rows, int capacity);

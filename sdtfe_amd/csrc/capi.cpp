@@ -415,6 +415,36 @@ int sdtfe_tess_triangles(const sdtfe_tess *t, int *rows, int capacity) {
     r[6] = tet[k];
     r[7] = 0;
   }
+  // Neighbours across shared edges (src/triangulate.c:358-392): slot 5 for
+  // edge (a, b), 4 for (a, c), 3 for (b, c); the reference compares all
+  // pairs, so of several triangles on one edge the highest-numbered other
+  // one is kept.  Here the edges are sorted by their vertex pair instead.
+  struct Edge {
+    int lo, hi, tri, slot;
+  };
+  std::vector<Edge> e;
+  e.reserve(3 * (size_t)f);
+  static const int pair[3][3] = {{0, 1, 5}, {0, 2, 4}, {1, 2, 3}};
+  for (int k = 0; k < f; ++k)
+    for (const auto &q : pair) {
+      const int u = rows[8 * (size_t)k + q[0]], v = rows[8 * (size_t)k + q[1]];
+      e.push_back({std::min(u, v), std::max(u, v), k, q[2]});
+    }
+  std::sort(e.begin(), e.end(), [](const Edge &x, const Edge &y) {
+    return x.lo != y.lo ? x.lo < y.lo : (x.hi != y.hi ? x.hi < y.hi : x.tri < y.tri);
+  });
+  for (size_t s0 = 0; s0 < e.size();) {
+    size_t s1 = s0;
+    while (s1 < e.size() && e[s1].lo == e[s0].lo && e[s1].hi == e[s0].hi) ++s1;
+    // sorted by tri within the group: the highest other is the last, or
+    // the one before it for the last itself
+    for (size_t a = s0; a < s1; ++a) {
+      const int best = e[s1 - 1].tri != e[a].tri ? e[s1 - 1].tri
+                                                  : (s1 - s0 > 1 ? e[s1 - 2].tri : -1);
+      if (best >= 0 && best != e[a].tri) rows[8 * (size_t)e[a].tri + e[a].slot] = best;
+    }
+    s0 = s1;
+  }
   return 0;
 }
 
