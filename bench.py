@@ -328,13 +328,16 @@ def bench_volume(a):
     log(f"[bench] volume {G}^3 x{ss}: {ms:.2f} ms/grid (kernel {kernel_ms:.2f} ms), "
         f"{st['tet_steps'] / rays:.1f} tets/ray")
     # algorithmic bytes: 88 per tet crossed, plus one read and one write of
-    # every cell each ray passes (the cells of its column inside the hull)
+    # every cell each ray passes (the cells of its column inside the hull);
+    # kernel time: HIP events around the launch sequence (with supersampling
+    # > 1 the ray kernel and the fold kernel)
     cell_visits = rays * G
     alg = st["tet_steps"] * BYTES_PER_STEP + 16 * cell_visits
     achieved = alg / (kernel_ms / 1e3) / 1e9
     workload = (f"clustered-{pos.shape[0]} {G}^3 volumetric grid, {ss}x{ss} rays per column, "
                 f"{ss} points per cell in z")
-    tr = measured_traffic("volume_fp64_kernel", workload)
+    kernel = ("volume_rays_kernel+volume_fold_kernel" if ss > 1 else "volume_fp64_kernel")
+    tr = measured_traffic(kernel, workload)
     cpu = None
     if not a.no_cpu:
         from oracle import oracle as O
@@ -364,7 +367,7 @@ def bench_volume(a):
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": None if tr is None else tr["bytes_per_launch"],
-                        "kernel": "volume_fp64_kernel",
+                        "kernel": kernel,
                         "bytes_per_step": BYTES_PER_STEP, "bytes_per_cell_visit": 16,
                         "tet_steps_per_launch": st["tet_steps"], "cell_visits": cell_visits,
                         "kernel_ms": round(kernel_ms, 3)},

@@ -6,6 +6,7 @@ terms in the reference's order, in FP64 with the reference's operations, so
 the grid equals the reference's (golden fixtures written by the reference)
 and the oracle's bit for bit, as do the counters.
 """
+import os
 import sys
 
 import numpy as np
@@ -172,3 +173,38 @@ def test_pydtfe_3d_functions(oracle):
         pydtfe.interpolate_field2grid(pos, val[:-1], cen, G, 1.0, 1.0)
     with pytest.raises(ValueError, match="not supported"):
         pydtfe.interpolate_field2grid(pos, val, cen, G, 0.5, 1.0)
+
+
+@pytest.mark.parametrize("G,ss", [(48, 2), (21, 3)])
+def test_volume_forms_agree(oracle, G, ss):
+    """With supersampling > 1 the grid comes from the ray kernels (a lane per
+    ray writes its terms, a lane per cell adds them in order); the column
+    kernel (a lane per column walks its rays in order), the redo path
+    (every column marked and recomputed by the column kernel) and bands of
+    5 grid rows give the same grid and counters bit for bit, and so does
+    the oracle."""
+    _say(f"[gpu] volume forms G={G} ss={ss}")
+    pos = workloads.clustered(65536, seed=21)
+    tetra = triangulate(pos)
+    pd = to_particle_data(pos)
+    want, st_o = oracle.volume(pd.copy(), tetra, G, 0.9, (0.5, 0.48, 0.52), ss, seed=6,
+                               p_mass=1.0, n_threads=8)
+    start = np.random.default_rng(2).random(G ** 3)
+    res = {}
+    with capi.Tessellation(pd, tetra) as t:
+        for form in ("", "columns", "redo", "bands"):
+            os.environ["SDTFE_VOLUME_FORM"] = form
+            os.environ["SDTFE_VOLUME_BAND_ROWS"] = "5" if form == "bands" else ""
+            try:
+                res[form] = t.volume(G, ss, grid=start.copy(), box_len=0.9,
+                                     center=(0.5, 0.48, 0.52), seed=6)
+            finally:
+                os.environ.pop("SDTFE_VOLUME_FORM", None)
+                os.environ.pop("SDTFE_VOLUME_BAND_ROWS", None)
+    want_acc, _ = oracle.volume(pd.copy(), tetra, G, 0.9, (0.5, 0.48, 0.52), ss, seed=6,
+                                p_mass=1.0, grid=start.copy(), n_threads=8)
+    for form, (grid, st) in res.items():
+        np.testing.assert_array_equal(grid, want_acc, err_msg=form)
+        for k in ("samples", "missed", "failed", "tet_steps", "perturb_steps"):
+            assert st[k] == st_o[k], (form, k)
+    assert np.isfinite(want).all()
