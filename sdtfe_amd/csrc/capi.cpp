@@ -37,7 +37,6 @@ void launch_trace_mixed(sdtfe_tess *t, const MapGeom &g, int n, const double *x,
                         const double *y, const int *entry, const uint64_t *pixel,
                         const uint32_t *sample, double *value, int *crossed);
 void launch_locate(sdtfe_tess *t, int n, const double *x, const double *y, int *entry);
-void launch_volume_fp64(sdtfe_tess *t, const VolGeom &v, double *grid);
 
 extern const bool kHaveMixed;
 
@@ -113,28 +112,6 @@ int check_volume_params(const sdtfe_volume_params *p) {
     return fail(SDTFE_EINVAL, "grid_dim * supersampling too large");
   if (!(p->box_len > 0)) return fail(SDTFE_EINVAL, "box_len must be > 0");
   return 0;
-}
-
-// src/dtfe.c:386-409, 686-695
-VolGeom make_vol_geom(const sdtfe_volume_params &p) {
-  VolGeom v{};
-  const unsigned ss = p.supersampling;
-  v.dxy = p.box_len / p.grid_dim;
-  v.ds = v.dxy / ss;
-  v.off_x = -(p.box_len / 2.0) + p.center[0];
-  v.off_y = -(p.box_len / 2.0) + p.center[1];
-  v.off_z = -(p.box_len / 2.0) + p.center[2];
-  v.lz = p.box_len;
-  const double dz = v.lz / p.grid_dim;
-  v.dzs = dz / ss;
-  v.ss_weight = 1. / (ss * ss * ss);
-  v.pert_step = (v.dxy / 2.0) / kPerturbMax;
-  v.seed = p.seed;
-  v.grid = p.grid_dim;
-  v.ss = (int)ss;
-  v.nzs = p.grid_dim * (int)ss;
-  v.max_steps = p.max_steps > 0 ? p.max_steps : SDTFE_MAX_STEPS_DEFAULT;
-  return v;
 }
 
 int run_volume(sdtfe_tess *t, const sdtfe_volume_params *p, double *grid, double *t_walk_ms) {

@@ -113,6 +113,10 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st)
 // A tessellation as sdtfe_tess_create makes it, uploading through st.
 int tess_create_staged(const double *particle_data, int n, const int *tetra_data, int nt,
                        float p_mass, Staging *st, sdtfe_tess **out);
+// volume.hip: the volumetric grid's geometry (src/dtfe.c:386-409,
+// 686-695) and its launch (the tessellation's stream)
+VolGeom make_vol_geom(const sdtfe_volume_params &p);
+void launch_volume_fp64(sdtfe_tess *t, const VolGeom &v, double *grid);
 // field.hip
 int build_field(sdtfe_tess *t, const double *particle_data, const int *tetra_data,
                 Staging *st);

@@ -24,6 +24,28 @@
 
 namespace sdtfe {
 
+// src/dtfe.c:386-409, 686-695
+VolGeom make_vol_geom(const sdtfe_volume_params &p) {
+  VolGeom v{};
+  const unsigned ss = p.supersampling;
+  v.dxy = p.box_len / p.grid_dim;
+  v.ds = v.dxy / ss;
+  v.off_x = -(p.box_len / 2.0) + p.center[0];
+  v.off_y = -(p.box_len / 2.0) + p.center[1];
+  v.off_z = -(p.box_len / 2.0) + p.center[2];
+  v.lz = p.box_len;
+  const double dz = v.lz / p.grid_dim;
+  v.dzs = dz / ss;
+  v.ss_weight = 1. / (ss * ss * ss);
+  v.pert_step = (v.dxy / 2.0) / kPerturbMax;
+  v.seed = p.seed;
+  v.grid = p.grid_dim;
+  v.ss = (int)ss;
+  v.nzs = p.grid_dim * (int)ss;
+  v.max_steps = p.max_steps > 0 ? p.max_steps : SDTFE_MAX_STEPS_DEFAULT;
+  return v;
+}
+
 // Fine points of one tet: k in [ceil((z_in - off_z - dzs/2) / dzs),
 // ceil((z_out - off_z - dzs/2) / dzs)) clamped to [0, nzs) (src/dtfe.c:
 // 482-490).  The clamps act on the double before its conversion, which
@@ -229,10 +251,12 @@ SDTFE_HD inline void column_ray(const VolGeom &v, int i, int j, int is, int js, 
                    (uint32_t)(is * v.ss + js));
 }
 
-// All rays of column (i, j) in order, adding into the grid.
-__device__ inline void walk_column(const Rec64 *__restrict__ rec, const EntryGrid &eg,
-                                   const VolGeom &v, double *grid, int i, int j, LineWindow &win,
-                                   bool pairs, LaneCounters &cnt) {
+// All rays of column (i, j) in order, adding into the grid; locate(x, y)
+// gives a ray's entry tet (-1 outside the hull's shadow).
+template <class Locate>
+SDTFE_HD inline void walk_column(const Rec64 *__restrict__ rec, const Locate &locate,
+                                 const VolGeom &v, double *grid, int i, int j, LineWindow &win,
+                                 bool pairs, LaneCounters &cnt) {
   double *col = grid + ((uint64_t)i * (uint64_t)v.grid + (uint64_t)j) * (uint64_t)v.grid;
   CellSink sink{win, col, v.grid, v.ss, pairs};
   for (int is = 0; is < v.ss; ++is)
@@ -241,7 +265,7 @@ __device__ inline void walk_column(const Rec64 *__restrict__ rec, const EntryGri
       uint64_t key;
       column_ray(v, i, j, is, js, x, y, key);
       cnt.samples++;
-      const int entry = locate_entry(eg, x, y);
+      const int entry = locate(x, y);
       if (entry < 0) {
         cnt.missed++;
         continue;
@@ -271,9 +295,11 @@ volume_fp64_kernel(const Rec64 *__restrict__ rec, EntryGrid eg, VolGeom v, doubl
   int i, j;
   block_patch((v.grid + 15) / 16, i, j);
   LaneCounters cnt;
-  if (i < v.grid && j < v.grid) walk_column(rec, eg, v, grid, i, j, win, pairs, cnt);
+  auto locate = [&](double x, double y) { return locate_entry(eg, x, y); };
+  if (i < v.grid && j < v.grid) walk_column(rec, locate, v, grid, i, j, win, pairs, cnt);
   cnt.flush(stats);
 }
+
 
 // Scratch of the ray kernels for grid rows i in [i0, i0 + rows): the rays
 // form a (rows*ss) x (G*ss) lattice, ray (X, Y) = (i*ss + is, j*ss + js)
@@ -289,56 +315,52 @@ struct RayScratch {
   int mark_all;   // every column to the column kernel (a check of that path)
 };
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void
-volume_rays_kernel(const Rec64 *__restrict__ rec, EntryGrid eg, VolGeom v, RayScratch s,
-                   unsigned long long *stats) {
+// Ray (X, Y) of the band (lattice position within it): its terms into its
+// scratch row, its range, and its column's mark when the row cannot be used;
+// locate(x, y) gives its entry tet.
+template <class Locate>
+SDTFE_HD inline void ray_terms(const Rec64 *__restrict__ rec, const Locate &locate,
+                               const VolGeom &v, const RayScratch &s, int X, int Y,
+                               LaneCounters &cnt) {
   const int gs = v.grid * v.ss;
-  int X, Y;  // within the band
-  block_patch((gs + 15) / 16, X, Y);
-  LaneCounters cnt;
-  if (X < s.rows * v.ss && Y < gs) {
-    const int i = s.i0 + X / v.ss, is = X % v.ss, j = Y / v.ss, js = Y % v.ss;
-    const uint64_t ray = (uint64_t)X * (uint64_t)gs + (uint64_t)Y;
-    double x, y;
-    uint64_t key;
-    column_ray(v, i, j, is, js, x, y, key);
-    cnt.samples++;
-    int2 range = make_int2(1, 0);
-    const int entry = locate_entry(eg, x, y);
-    if (entry < 0) {
-      cnt.missed++;
-    } else {
-      const bool pairs = (v.nzs & 1) == 0 && ((uintptr_t)s.terms & 15) == 0;
-      TermSink sink{s.terms + ray * (uint64_t)v.nzs, pairs};
-      const int crossed = ray_walk64(rec, entry, x, y, v, key, sink, cnt);
-      cnt.tet_steps += (unsigned)crossed;
-      if (crossed == 0) cnt.failed++;
-      if (sink.bad || s.mark_all) {
-        const int c = (i - s.i0) * v.grid + j;
-        if (atomicOr(s.mark + c, 1) == 0) s.redo[atomicAdd(s.n_redo, 1)] = i * v.grid + j;
-      } else if (sink.first >= 0) {
-        range = make_int2(sink.first, sink.last);
+  const int i = s.i0 + X / v.ss, is = X % v.ss, j = Y / v.ss, js = Y % v.ss;
+  const uint64_t ray = (uint64_t)X * (uint64_t)gs + (uint64_t)Y;
+  double x, y;
+  uint64_t key;
+  column_ray(v, i, j, is, js, x, y, key);
+  cnt.samples++;
+  int2 range = make_int2(1, 0);
+  const int entry = locate(x, y);
+  if (entry < 0) {
+    cnt.missed++;
+  } else {
+    const bool pairs = (v.nzs & 1) == 0 && ((uintptr_t)s.terms & 15) == 0;
+    TermSink sink{s.terms + ray * (uint64_t)v.nzs, pairs};
+    const int crossed = ray_walk64(rec, entry, x, y, v, key, sink, cnt);
+    cnt.tet_steps += (unsigned)crossed;
+    if (crossed == 0) cnt.failed++;
+    if (sink.bad || s.mark_all) {
+      const int c = (i - s.i0) * v.grid + j;
+#if defined(__HIP_DEVICE_COMPILE__)
+      if (atomicOr(s.mark + c, 1) == 0) s.redo[atomicAdd(s.n_redo, 1)] = i * v.grid + j;
+#else
+      if (s.mark[c] == 0) {
+        s.mark[c] = 1;
+        s.redo[(*s.n_redo)++] = i * v.grid + j;
       }
+#endif
+    } else if (sink.first >= 0) {
+      range = make_int2(sink.first, sink.last);
     }
-    s.range[ray] = range;
   }
-  cnt.flush(stats);
+  s.range[ray] = range;
 }
 
-// One lane per cell (i, j, iz) of the band: the cell's terms, ray by ray in
-// (is, js) order and each ray's fine points upward, added into the grid.
-__global__ __launch_bounds__(256) void volume_fold_kernel(VolGeom v, RayScratch s,
-                                                          double *grid) {
-  const uint64_t cell = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t g = (uint64_t)v.grid;
-  if (cell >= (uint64_t)s.rows * g * g) return;
-  const int iz = (int)(cell % g);
-  const uint64_t c = cell / g;  // column within the band
-  const int j = (int)(c % g), ib = (int)(c / g);
-  if (s.mark[c]) return;
-  const uint64_t gs = g * (uint64_t)v.ss;
-  double *out = grid + ((uint64_t)(s.i0 + ib) * g + (uint64_t)j) * g + (uint64_t)iz;
-  double val = *out;
+// Cell (i0 + ib, j, iz) of the band: its terms, ray by ray in (is, js) order
+// and each ray's fine points upward, added onto val.
+SDTFE_HD inline double fold_cell(const VolGeom &v, const RayScratch &s, int ib, int j, int iz,
+                                 double val) {
+  const uint64_t gs = (uint64_t)v.grid * (uint64_t)v.ss;
   for (int is = 0; is < v.ss; ++is)
     for (int js = 0; js < v.ss; ++js) {
       const uint64_t ray = (uint64_t)(ib * v.ss + is) * gs + (uint64_t)(j * v.ss + js);
@@ -351,7 +373,33 @@ __global__ __launch_bounds__(256) void volume_fold_kernel(VolGeom v, RayScratch 
         if (__builtin_bit_cast(uint64_t, t) != kNoTerm) val = val + t;
       }
     }
-  *out = val;
+  return val;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void
+volume_rays_kernel(const Rec64 *__restrict__ rec, EntryGrid eg, VolGeom v, RayScratch s,
+                   unsigned long long *stats) {
+  const int gs = v.grid * v.ss;
+  int X, Y;  // within the band
+  block_patch((gs + 15) / 16, X, Y);
+  LaneCounters cnt;
+  auto locate = [&](double x, double y) { return locate_entry(eg, x, y); };
+  if (X < s.rows * v.ss && Y < gs) ray_terms(rec, locate, v, s, X, Y, cnt);
+  cnt.flush(stats);
+}
+
+// One lane per cell (i, j, iz) of the band, unless its column is marked.
+__global__ __launch_bounds__(256) void volume_fold_kernel(VolGeom v, RayScratch s,
+                                                          double *grid) {
+  const uint64_t cell = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t g = (uint64_t)v.grid;
+  if (cell >= (uint64_t)s.rows * g * g) return;
+  const int iz = (int)(cell % g);
+  const uint64_t c = cell / g;  // column within the band
+  const int j = (int)(c % g), ib = (int)(c / g);
+  if (s.mark[c]) return;
+  double *out = grid + ((uint64_t)(s.i0 + ib) * g + (uint64_t)j) * g + (uint64_t)iz;
+  *out = fold_cell(v, s, ib, j, iz, *out);
 }
 
 // Marked columns of the band, by the column kernel's order of work.
@@ -363,9 +411,10 @@ volume_redo_kernel(const Rec64 *__restrict__ rec, EntryGrid eg, VolGeom v, RaySc
   win.w = lds + threadIdx.x * kWinStride;
   const bool pairs = (v.grid & 1) == 0 && ((uintptr_t)grid & 15) == 0;
   LaneCounters cnt;  // the ray kernel counted these rays
+  auto locate = [&](double x, double y) { return locate_entry(eg, x, y); };
   const int n = *s.n_redo;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
-    walk_column(rec, eg, v, grid, s.redo[q] / v.grid, s.redo[q] % v.grid, win, pairs, cnt);
+    walk_column(rec, locate, v, grid, s.redo[q] / v.grid, s.redo[q] % v.grid, win, pairs, cnt);
 }
 
 __global__ void volume_band_reset_kernel(int *mark, int n, int *n_redo) {
