@@ -150,7 +150,8 @@ def tessellation(name, pos, cache):
     """Host Qhull tessellation, optionally cached in an .npz (e.g. under /tmp
     between profiling passes of one session).  Above 2M particles the
     Delaunay tetrahedralisation is assembled from x-slabs in parallel (one
-    Qhull run over 8M particles takes many minutes on one core)."""
+    Qhull run over 8M particles takes many minutes on one core), with the
+    orientation and neighbour slots assembled on the GPU."""
     from sdtfe_amd.tessellate import triangulate
     from sdtfe_amd.tessellate_chunked import triangulate_chunked
     if cache and os.path.exists(cache):
@@ -159,7 +160,7 @@ def tessellation(name, pos, cache):
                 return z["tetra"]
     if pos.shape[0] > 2_000_000:
         workers = min(16, os.cpu_count() or 1)
-        tetra = triangulate_chunked(pos, slabs=16, workers=workers, log=log)
+        tetra = triangulate_chunked(pos, slabs=16, workers=workers, log=log, gpu=True)
     else:
         tetra = triangulate(pos)
     if cache:

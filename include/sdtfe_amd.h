@@ -132,6 +132,17 @@ typedef struct {
   double t_setup_ms;               /* upload + field + entry structure */
 } sdtfe_tess_info;
 
+/* The reference's tetra_data (T x 8: A, B, C, D, then the tet across the
+ * face opposite each, -1 on the hull) from a Delaunay tetrahedralisation's
+ * simplices (T x 4 vertex ids, in the order Qhull lists a facet's vertices)
+ * and particle_data (N x 4, columns 0-2 read), assembled on the GPU as
+ * triangulate assembles Qhull's facets (src/triangulate.c:56-175): B and C
+ * swapped where (AB x AC) . AD > 0, the reference's expression and
+ * operation order.  SDTFE_ETOPO when a vertex id is out of range or a face
+ * is shared by more than two tets. */
+int sdtfe_tetra_from_simplices(const double *particle_data, int n_particles,
+                               const int *simplices, int n_tetra, int *tetra_data);
+
 const char *sdtfe_last_error(void);
 void sdtfe_set_seed(uint64_t seed);
 uint64_t sdtfe_get_seed(void);

@@ -32,6 +32,7 @@ UNITS = {
     "walk_mixed.hip": ["-ffp-contract=off"],
     "batch.hip": ["-ffp-contract=off"],
     "volume.hip": ["-ffp-contract=off"],
+    "adapter.hip": ["-ffp-contract=off"],
     "capi.cpp": ["-x", "hip", "-ffp-contract=off"],
     "server.cpp": ["-x", "hip", "-ffp-contract=off"],
 }

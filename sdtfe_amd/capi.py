@@ -93,6 +93,7 @@ _SIGS = {
     "sdtfe_map": (C.c_int, [_vp, C.POINTER(MapParams), _dp, _vp, C.POINTER(Stats)]),
     "sdtfe_map_device": (C.c_int, [_vp, C.POINTER(MapParams), _vp, C.POINTER(Stats)]),
     "sdtfe_tess_sync": (C.c_int, [_vp]),
+    "sdtfe_tetra_from_simplices": (C.c_int, [_dp, C.c_int, _ip, C.c_int, _ip]),
     "sdtfe_volume": (C.c_int, [_vp, C.POINTER(VolumeParams), _dp, C.POINTER(Stats)]),
     "sdtfe_volume_device": (C.c_int, [_vp, C.POINTER(VolumeParams), _vp, C.POINTER(Stats)]),
     "sdtfe_batch_create": (C.c_int, [C.POINTER(_vp)]),
@@ -473,6 +474,24 @@ def compute_3d_density(particle_data, tetra_data, grid_dim, box_len, rho, p_mass
     lib().compute_3d_density(particle_data, particle_data.shape[0], tet, tet.shape[0],
                              grid_dim, box_len, rho, p_mass, center[0], center[1], center[2],
                              supersampling)
+
+
+def tetra_from_simplices(particle_data, simplices) -> np.ndarray:
+    """tetra_data (T x 8 int32, the reference layout) from Delaunay
+    simplices (T x 4) and particle_data (N x 4 or N x 3 float64), assembled
+    on the GPU: orientation and neighbour slots as triangulate makes them."""
+    p = np.asarray(particle_data, dtype=np.float64)
+    if p.ndim != 2 or p.shape[1] not in (3, 4):
+        raise ValueError("particle_data must be N x 3 or N x 4")
+    if p.shape[1] == 3:
+        p = np.concatenate([p, np.zeros((p.shape[0], 1))], axis=1)
+    p = np.ascontiguousarray(p)
+    simp = _c64(simplices, np.int32)
+    if simp.ndim != 2 or simp.shape[1] != 4:
+        raise ValueError("simplices must be T x 4")
+    out = np.empty((simp.shape[0], 8), dtype=np.int32)
+    check(lib().sdtfe_tetra_from_simplices(p, p.shape[0], simp, simp.shape[0], out))
+    return out
 
 
 def set_seed(seed: int) -> None:

@@ -597,6 +597,11 @@ int sdtfe_volume_device(sdtfe_tess *t, const sdtfe_volume_params *p, double *gri
   return 0;
 }
 
+int sdtfe_tetra_from_simplices(const double *particle_data, int n_particles,
+                               const int *simplices, int n_tetra, int *tetra_data) {
+  return tetra_from_simplices(particle_data, n_particles, simplices, n_tetra, tetra_data);
+}
+
 int sdtfe_tess_sync(sdtfe_tess *t) {
   if (!t) return fail(SDTFE_EINVAL, "null tessellation");
   SDTFE_HIP(hipStreamSynchronize(t->stream));

@@ -117,6 +117,9 @@ int tess_create_staged(const double *particle_data, int n, const int *tetra_data
 // 686-695) and its launch (the tessellation's stream)
 VolGeom make_vol_geom(const sdtfe_volume_params &p);
 void launch_volume_fp64(sdtfe_tess *t, const VolGeom &v, double *grid);
+// adapter.hip: the reference's tetra_data from Delaunay simplices
+int tetra_from_simplices(const double *particle_data, int n, const int *simplices, int nt,
+                         int *tetra_data);
 // field.hip
 int build_field(sdtfe_tess *t, const double *particle_data, const int *tetra_data,
                 Staging *st);

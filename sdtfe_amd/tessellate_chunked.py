@@ -101,7 +101,7 @@ def _neighbors(simp: np.ndarray) -> np.ndarray:
     """neighbors[t, k]: the tet across the face opposite vertex k (-1 on the
     hull); raises if a face has more than two tets."""
     t = simp.shape[0]
-    s = np.sort(simp, axis=1)
+    s = np.sort(simp, axis=1).astype(np.int64)  # the packed keys need 64 bits
     bits = max(1, int(s.max()).bit_length())
     # face opposite vertex k of the sorted row, as two keys (the first two
     # vertices packed, the third); rows keep their original slot order
@@ -131,11 +131,13 @@ def _neighbors(simp: np.ndarray) -> np.ndarray:
 
 def triangulate_chunked(pos: np.ndarray, slabs: int = 16, workers: int = 16,
                         margin: float | None = None, shell: float | None = None,
-                        log=print) -> np.ndarray:
+                        log=print, gpu: bool = False) -> np.ndarray:
     """tetra_data (reference layout) of pos's Delaunay tetrahedralisation,
     assembled from x-slabs in parallel (see the module docstring).  margin:
     the slabs' overlap in x (default 8 mean spacings); shell: the depth of
     the y and z boundary layers every slab includes (default 3 spacings).
+    gpu: orientation and neighbour slots by the library's GPU assembly
+    (sdtfe_tetra_from_simplices) instead of on the host; the same array.
     Raises ValueError when the slabs' tets do not tile the hull."""
     global _POS, _TREE
     from scipy.spatial import ConvexHull, cKDTree
@@ -170,14 +172,23 @@ def triangulate_chunked(pos: np.ndarray, slabs: int = 16, workers: int = 16,
             log(f"[tess] slab {k}: {s.shape[0]} tets ({n_unc} checked against all points, "
                 f"{n_drop} dropped), {time.perf_counter() - t0:.1f} s")
     simp = np.concatenate(parts)
-    nb = _neighbors(simp)
-    log(f"[tess] neighbours of {simp.shape[0]} tets, {time.perf_counter() - t0:.1f} s")
+    if gpu:
+        from .capi import SdtfeError, tetra_from_simplices
+        try:
+            tetra = tetra_from_simplices(p, simp)
+        except SdtfeError as e:
+            raise ValueError(f"slab tets do not form a tessellation: {e}") from e
+        n_hull = int((tetra[:, 4:] < 0).sum())
+    else:
+        nb = _neighbors(simp)
+        n_hull = int((nb < 0).sum())
+    log(f"[tess] neighbours of {simp.shape[0]} tets ({'GPU' if gpu else 'host'}), "
+        f"{time.perf_counter() - t0:.1f} s")
     vol = np.abs(circumspheres(p, simp)[2]).sum() / 6.0
     hull = ConvexHull(p)
-    n_hull = int((nb < 0).sum())
     log(f"[tess] {simp.shape[0]} tets; volume {vol:.12g} vs hull {hull.volume:.12g}; "
         f"{n_hull} hull faces vs {hull.simplices.shape[0]}; {time.perf_counter() - t0:.1f} s")
     if abs(vol - hull.volume) > 1e-9 * hull.volume or n_hull != hull.simplices.shape[0]:
         raise ValueError("slab tets do not tile the convex hull")
     _POS = _TREE = None
-    return orient_and_link(p, simp, nb)
+    return tetra if gpu else orient_and_link(p, simp, nb)

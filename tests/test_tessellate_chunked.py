@@ -32,3 +32,18 @@ def test_slabs_that_do_not_tile_raise():
     pos = workloads.jittered_lattice(24, 24, 12, seed=4)
     with pytest.raises(ValueError):
         triangulate_chunked(pos, slabs=4, workers=2, margin=0.0, shell=0.0, log=lambda m: None)
+
+
+def test_face_matching_of_int32_simplices():
+    """The host face matching packs two vertex ids into one key (id << bits
+    | id): with Qhull's int32 simplices over more than 2^16 points that
+    overflows 32 bits, so the keys must be built in 64 bits to find
+    exactly Qhull's neighbours."""
+    from scipy.spatial import Delaunay
+
+    from sdtfe_amd.tessellate import QHULL_OPTIONS
+    from sdtfe_amd.tessellate_chunked import _neighbors
+    pos = np.random.default_rng(9).random((70000, 3))
+    tri = Delaunay(pos, qhull_options=QHULL_OPTIONS)
+    assert tri.simplices.dtype == np.int32 and tri.simplices.max() >= 1 << 16
+    np.testing.assert_array_equal(_neighbors(tri.simplices), tri.neighbors)
