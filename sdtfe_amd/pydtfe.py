@@ -135,7 +135,7 @@ def compute_surface_density(particle_pos, center, grid_dim, sample_factor, box_w
     if rotation_angle is not None:
         rotate3d(pd, n_shuffle, rotation_angle, center)
     pd = np.ascontiguousarray(pd[:n_shuffle])
-    tetra = triangulate(pd)
+    tetra = triangulate(pd, gpu=True)
     g = int(grid_dim)
     rho = np.zeros(g * g)
     capi.set_seed(seed)
@@ -169,7 +169,7 @@ def compute_3d_density(particle_pos, center, grid_dim, sample_factor, box_length
     shuffle_buff(pd, n, n_shuffle, seed)
     mass = float(particle_mass) / float(np.float32(sample_factor))
     pd = np.ascontiguousarray(pd[:n_shuffle])
-    tetra = triangulate(pd)
+    tetra = triangulate(pd, gpu=True)
     g = int(grid_dim)
     rho = np.zeros(g ** 3)
     capi.set_seed(seed)
@@ -192,7 +192,7 @@ def interpolate_field2grid(particle_pos, particle_value, center, grid_dim, sampl
     pd = convert_to_qhdata(pos)
     if np.float32(sample_factor) != np.float32(1.0):
         raise ValueError("`sample_factor != 1.0f` not supported at the moment")
-    tetra = triangulate(pd)
+    tetra = triangulate(pd, gpu=True)
     pd[:, 3] = val.astype(np.float64)
     g = int(grid_dim)
     grid = np.zeros(g ** 3)
@@ -208,7 +208,7 @@ def compute_particle_density(particle_pos, particle_mass):
     python/pydtfe.cpp:165-191."""
     pos = _positions(particle_pos)
     pd = convert_to_qhdata(pos)
-    tetra = triangulate(pd)
+    tetra = triangulate(pd, gpu=True)
     ptr = C.c_void_p(pd.ctypes.data)
     capi.lib().pre_compute_vol(C.byref(ptr), pd.shape[0], tetra, tetra.shape[0],
                                float(np.float32(particle_mass)))

@@ -55,12 +55,16 @@ def orient_and_link(pos: np.ndarray, simplices: np.ndarray,
     return out
 
 
-def triangulate(pos: np.ndarray) -> np.ndarray:
-    """Delaunay-tetrahedralise ``pos`` (N x 3 or N x 4 float64) on the host.
+def triangulate(pos: np.ndarray, gpu: bool = False) -> np.ndarray:
+    """Delaunay-tetrahedralise ``pos`` (N x 3 or N x 4 float64) with Qhull on
+    the host.
 
     Returns ``tetra_data`` in the reference layout.  Mirrors ``triangulate``
-    (src/triangulate.c:8-181).  Raises ``ValueError`` for fewer than 4 points
-    or a degenerate (flat) point set, where Qhull fails.
+    (src/triangulate.c:8-181).  gpu: orientation and neighbour slots are
+    assembled on the GPU (sdtfe_tetra_from_simplices; the same array)
+    instead of from Qhull's neighbours on the host.  Raises ``ValueError``
+    for fewer than 4 points or a degenerate (flat) point set, where Qhull
+    fails.
     """
     from scipy.spatial import Delaunay  # host dependency, imported lazily
 
@@ -71,6 +75,9 @@ def triangulate(pos: np.ndarray) -> np.ndarray:
         tri = Delaunay(p, qhull_options=QHULL_OPTIONS)
     except Exception as exc:  # scipy raises QhullError
         raise ValueError(f"Qhull failed: {exc}") from exc
+    if gpu:
+        from .capi import tetra_from_simplices
+        return tetra_from_simplices(p, tri.simplices)
     return orient_and_link(p, tri.simplices, tri.neighbors)
 
 

@@ -262,3 +262,29 @@ def test_small_and_ragged_maps(oracle, G, S):
     assert np.isfinite(rho).all() and _rel(r32, rho) <= 1e-5
     if S == 0:
         assert not rho.any() and not r32.any() and not navg.any()
+
+
+@pytest.mark.parametrize("factor,rot", [(1.0, None), (0.6, [0.3, -0.7, 1.1])])
+def test_pydtfe_surface_density(oracle, factor, rot):
+    """The pydtfe mirror end to end (float32 positions, shuffle, mass /
+    sample_factor, rotation, Qhull with the GPU tet assembly, then the
+    drop-in compute_density) against the oracle's compute_density on the
+    same pre-processed particles."""
+    from sdtfe_amd import pydtfe
+    _say(f"[gpu] pydtfe.compute_surface_density factor={factor} rot={rot is not None}")
+    pos = workloads.clustered(30000, seed=17)
+    cen, G, S = [0.5, 0.45, 0.5], 96, 6
+    got = pydtfe.compute_surface_density(pos, cen, G, factor, 0.9, 0.7, 2.0, 0.02, S,
+                                         rotation_angle=rot, seed=5)
+    assert got.shape == (G, G)
+    pd = pydtfe.convert_to_qhdata(np.asarray(pos, dtype=np.float32, order="F"))
+    n_sh = int(np.float32(pos.shape[0]) * np.float32(factor))
+    pydtfe.shuffle_buff(pd, pos.shape[0], n_sh, 5)
+    if rot is not None:
+        pydtfe.rotate3d(pd, n_sh, rot, cen)
+    pd = np.ascontiguousarray(pd[:n_sh])
+    mass = float(np.float32(2.0 / float(np.float32(factor))))
+    want, _ = oracle.compute_density(pd, triangulate(pd), G, 0.9, 0.7, mass, cen, S, 0.02,
+                                     5, n_threads=8)
+    assert _rel(got.ravel(), want) <= RTOL_FP64
+    np.testing.assert_array_equal(got.ravel(), want)
