@@ -101,20 +101,24 @@ map_mixed_kernel(const Rec32 *__restrict__ rec32, const Rec64 *__restrict__ rec6
   cnt.flush(stats);
 }
 
-// Second pass: one wave per queued pixel, grid-stride over the queue.
+// Second pass: one wave per queued pixel, grid-stride over the queue; does
+// nothing when the queue holds more than max_entries pixels (the host then
+// runs the per-sample form) or overflowed (the host repeats the map), so it
+// can be queued before the host has read the queue's length.
 template <bool kWide>
 __global__ __launch_bounds__(256) void map_deferred_kernel(const Rec32 *__restrict__ rec32,
                                                           const Rec64 *__restrict__ rec64,
                                                           const FaceRec *__restrict__ frec,
                                                           EntryGrid eg, MapGeom g, MixedConsts mc,
                                                           double *out, int *navg, int accumulate,
-                                                          DeferQueue dq,
+                                                          DeferQueue dq, int max_entries,
                                                           unsigned long long *stats) {
+  const int n = *dq.count;
+  if (n > max_entries || n > dq.cap) return;  // the same for every thread
   __shared__ uint8_t faces[kFaceCodes];
   load_faces(faces);
   const DeferCtx c{rec32, rec64, frec, faces, eg, g, mc.flt};
   LaneCounters cnt;
-  const int n = min(*dq.count, dq.cap);
   const int waves = gridDim.x * (blockDim.x >> 6);
   for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
     int n_ok;
@@ -335,8 +339,10 @@ void launch_first_pass(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, d
 
 // The queue holds a quarter of the pixels; a map that overflows it is run
 // again with room for all of them (an accumulating map gets that room from
-// the start, since its first pass cannot be repeated).  The second pass
-// starts once the queue's length is known.
+// the start, since its first pass cannot be repeated).  The per-pixel second
+// pass is queued behind the first and acts only on a short queue; the host
+// reads the queue's length meanwhile and runs the per-sample form for a long
+// one.
 int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
   const size_t npix = (size_t)g.grid * (size_t)g.grid;
   if (npix > (size_t)INT_MAX) return fail(SDTFE_EINVAL, "map too large for the mixed walk");
@@ -351,6 +357,15 @@ int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
     SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
     launch_first_pass(t, g, mc, out, navg, accumulate, dq);
     SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
+    const int max_wave = defer_wave_entries();
+    if (wide)
+      map_deferred_kernel<true><<<1024, 256, 0, t->stream>>>(t->rec32, t->rec64, t->facerec,
+                                                             t->grid, g, mc, out, navg,
+                                                             accumulate, dq, max_wave, t->stats);
+    else
+      map_deferred_kernel<false><<<1024, 256, 0, t->stream>>>(t->rec32, t->rec64, t->facerec,
+                                                              t->grid, g, mc, out, navg,
+                                                              accumulate, dq, max_wave, t->stats);
     SDTFE_HIP(hipGetLastError());
     int count = 0;
     SDTFE_HIP(hipMemcpyAsync(&count, t->defer_count, sizeof(int), hipMemcpyDeviceToHost,
@@ -363,17 +378,8 @@ int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
       if ((rc = ensure_defer(t, npix))) return rc;
       continue;
     }
-    if (count == 0) return 0;
-    if (count > defer_wave_entries())
-      return defer_by_sample(t, g, mc, out, navg, accumulate, count, wide);
-    if (wide)
-      map_deferred_kernel<true><<<1024, 256, 0, t->stream>>>(
-          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
-    else
-      map_deferred_kernel<false><<<1024, 256, 0, t->stream>>>(
-          t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
-    SDTFE_HIP(hipGetLastError());
-    return 0;
+    if (count > max_wave) return defer_by_sample(t, g, mc, out, navg, accumulate, count, wide);
+    return 0;  // done by the per-pixel pass above (or nothing queued)
   }
 }
 
