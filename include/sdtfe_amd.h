@@ -240,8 +240,7 @@ int sdtfe_batch_run(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
  * are on the GPU join the next launch.  Each map is identical alone and in a
  * mix.  rho (host, grid_dim^2) is accumulated into when the request
  * completes; sdtfe_server_wait blocks until then.  sdtfe_server_destroy
- * drains all requests first.  A mixed-precision server takes tessellations
- * of up to 2^25 tets (SDTFE_EINVAL at submit beyond; sdtfe_map takes any). */
+ * drains all requests first. */
 typedef struct sdtfe_server sdtfe_server;
 typedef struct {
   int64_t hbm_budget_bytes;

@@ -75,7 +75,7 @@ constexpr uint64_t kSlotNone = 0x7ff8dead00000001ull;
 constexpr uint64_t kSlotHanded = 0x7ff8dead00000002ull;
 
 // One sample of a pixel: value and tet count, false if handed over.
-template <bool kMixed>
+template <bool kMixed, bool kWide>
 __device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int px, int py,
                                   uint64_t pix, int s, double &v, int &crossed,
                                   LaneCounters &cnt) {
@@ -94,9 +94,9 @@ __device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int
   if (kMixed) {
     Resume rs;  // the second pass walks the sample again and resumes there
     const bool done = it.mc.flt.full_slab
-                          ? walk_mixed<true, false>(it.rec32, it.rec64, it.facerec, faces, entry,
+                          ? walk_mixed<true, kWide>(it.rec32, it.rec64, it.facerec, faces, entry,
                                                     xt, yt, g, it.mc.flt, v, crossed, cnt, rs)
-                          : walk_mixed<false, false>(it.rec32, it.rec64, it.facerec, faces, entry,
+                          : walk_mixed<false, kWide>(it.rec32, it.rec64, it.facerec, faces, entry,
                                                      xt, yt, g, it.mc.flt, v, crossed, cnt, rs);
     if (!done) return false;
   } else {
@@ -107,8 +107,9 @@ __device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int
   return true;
 }
 
-// The persistent loop; lds: per-sample values for lanes-per-pixel > 1.
-template <bool kMixed>
+// The persistent loop; lds: per-sample values for lanes-per-pixel > 1;
+// kWide: 64-bit face-record offsets (tessellations beyond 2^25 tets).
+template <bool kMixed, bool kWide>
 __device__ __forceinline__ void batch_body(const BatchItem *__restrict__ items,
                                            const BatchHdr *__restrict__ hdr, int *counters,
                                            unsigned long long *item_stats, double *lds,
@@ -160,7 +161,7 @@ __device__ __forceinline__ void batch_body(const BatchItem *__restrict__ items,
       for (int s = j; s < g.samples; s += L) {
         double v;
         int crossed;
-        const bool done = sample_one<kMixed>(it, faces, px, py, pix, s, v, crossed, cnt);
+        const bool done = sample_one<kMixed, kWide>(it, faces, px, py, pix, s, v, crossed, cnt);
         if (L == 1) {
           if (done)
             ps.add(v, crossed);  // a miss or a failed walk adds 0 (src/dtfe.c:626-636)
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(256) void batch_kernel_fp64(const BatchItem *__rest
                                                          unsigned long long *item_stats) {
   extern __shared__ double lds[];
   __shared__ int s_tile;
-  batch_body<false>(items, hdr, counters, item_stats, lds, s_tile, nullptr);
+  batch_body<false, false>(items, hdr, counters, item_stats, lds, s_tile, nullptr);
 }
 
 // Built for 4 waves per SIMD: the persistent loop holds more state than the
@@ -217,24 +218,26 @@ __global__ __launch_bounds__(256) void batch_kernel_fp64(const BatchItem *__rest
 #ifndef SDTFE_BATCH_MIXED_WAVES
 #define SDTFE_BATCH_MIXED_WAVES 4
 #endif
+template <bool kWide>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SDTFE_BATCH_MIXED_WAVES))) void
 batch_kernel_mixed(const BatchItem *__restrict__ items, const BatchHdr *__restrict__ hdr,
                    int *counters, unsigned long long *item_stats) {
   extern __shared__ double lds[];
   __shared__ int s_tile;
   __shared__ uint8_t faces[kFaceCodes];
-  batch_body<true>(items, hdr, counters, item_stats, lds, s_tile, faces);
+  batch_body<true, kWide>(items, hdr, counters, item_stats, lds, s_tile, faces);
 }
 
-template <bool kMixed>
+template <bool kMixed, bool kWide>
 constexpr auto batch_kernel() {
   if constexpr (kMixed)
-    return batch_kernel_mixed;
+    return batch_kernel_mixed<kWide>;
   else
     return batch_kernel_fp64;
 }
 
 // Second pass of a mixed batch: one wave per queued pixel of any item.
+template <bool kWide>
 __global__ __launch_bounds__(256) void batch_deferred_kernel(const BatchItem *__restrict__ items,
                                                             const BatchHdr *__restrict__ hdr,
                                                             const int *counters,
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(256) void batch_deferred_kernel(const BatchItem *__
     LaneCounters cnt;
     int n_ok;
     const DeferCtx c{it.rec32, it.rec64, it.facerec, faces, it.grid, it.geom, it.mc.flt};
-    const double sum = finish_deferred_wave<false>(d, c, n_ok, cnt);
+    const double sum = finish_deferred_wave<kWide>(d, c, n_ok, cnt);
     if ((threadIdx.x & 63) == 0) it.out[d->pix] += n_ok > 0 ? sum / n_ok : 0.0;
     cnt.flush(item_stats + (size_t)d->item * ST_COUNT);  // the wave's lanes share the item
   }
@@ -274,10 +277,10 @@ struct sdtfe_batch {
   // captured launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  bool graph_mixed = false;
+  bool graph_mixed = false, graph_wide = false;
   int graph_blocks = 0;
   size_t graph_lds = 0;
-  int blocks_fp64 = 0, blocks_mixed = 0;
+  int blocks_fp64 = 0, blocks_mixed = 0, blocks_wide = 0;
   sdtfe::Deferred *d_defer = nullptr;  // mixed walk: pixels for the FP64 pass
   size_t defer_cap = 0;
 };
@@ -313,11 +316,11 @@ int ensure_cap(sdtfe_batch *b, int n) {
   return 0;
 }
 
-template <bool kMixed>
+template <bool kMixed, bool kWide>
 int blocks_for(sdtfe_batch *, size_t lds) {
   int per_cu = 0;
-  SDTFE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<kMixed>(), 256,
-                                                         lds));
+  SDTFE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<kMixed, kWide>(),
+                                                         256, lds));
   int dev = 0;
   SDTFE_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
@@ -326,21 +329,22 @@ int blocks_for(sdtfe_batch *, size_t lds) {
 }
 
 // Capture reset + persistent kernel on the batch's stream.
-template <bool kMixed>
+template <bool kMixed, bool kWide>
 int capture(sdtfe_batch *b, int blocks, size_t lds) {
   drop_graph(b);
   SDTFE_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   batch_reset_kernel<<<64, 256, 0, b->stream>>>(b->d_hdr, b->d_counters, b->d_stats,
                                                 ST_COUNT * b->cap);
-  batch_kernel<kMixed>()<<<blocks, 256, lds, b->stream>>>(b->d_items, b->d_hdr, b->d_counters,
-                                                         b->d_stats);
+  batch_kernel<kMixed, kWide>()<<<blocks, 256, lds, b->stream>>>(b->d_items, b->d_hdr,
+                                                                b->d_counters, b->d_stats);
   if (kMixed)
-    batch_deferred_kernel<<<1024, 256, 0, b->stream>>>(b->d_items, b->d_hdr, b->d_counters,
-                                                       b->d_stats);
+    batch_deferred_kernel<kWide><<<1024, 256, 0, b->stream>>>(b->d_items, b->d_hdr,
+                                                              b->d_counters, b->d_stats);
   hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
   if (e != hipSuccess) return hip_check(e, "hipStreamEndCapture");
   SDTFE_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   b->graph_mixed = kMixed;
+  b->graph_wide = kWide;
   b->graph_blocks = blocks;
   b->graph_lds = lds;
   return 0;
@@ -380,11 +384,11 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
   }
   if (prec != SDTFE_PREC_FP64 && prec != SDTFE_PREC_MIXED)
     return fail(SDTFE_EINVAL, "unknown precision");
-  // the batched mixed walk addresses face records with 32-bit offsets
+  // face records of all items addressed with 64-bit offsets when any item
+  // needs them
+  bool wide = false;
   if (prec == SDTFE_PREC_MIXED)
-    for (int i = 0; i < n; ++i)
-      if ((int64_t)tess[i]->nt > kFaceRecNarrowTets)
-        return fail(SDTFE_EINVAL, "tessellation too large for a batched mixed map; map it alone");
+    for (int i = 0; i < n; ++i) wide |= wide_records(tess[i]->nt);
   int rc = ensure_cap(b, n);
   if (rc) return rc;
   const auto t0 = std::chrono::steady_clock::now();
@@ -447,12 +451,18 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
   h.total_tiles = tiles;
   for (int x = 0; x <= kXcd; ++x) h.share[x] = (int)((int64_t)tiles * x / kXcd);
 
-  int &blocks = mixed ? b->blocks_mixed : b->blocks_fp64;
-  if (!blocks) blocks = mixed ? blocks_for<true>(b, kLdsBytes) : blocks_for<false>(b, kLdsBytes);
+  int &blocks = !mixed ? b->blocks_fp64 : (wide ? b->blocks_wide : b->blocks_mixed);
+  if (!blocks)
+    blocks = !mixed ? blocks_for<false, false>(b, kLdsBytes)
+                    : (wide ? blocks_for<true, true>(b, kLdsBytes)
+                            : blocks_for<true, false>(b, kLdsBytes));
   if (blocks <= 0) return fail(SDTFE_EHIP, "occupancy query failed");
   // one LDS size for every launch, so one captured graph serves every batch
-  if (!b->exec || b->graph_mixed != mixed || b->graph_blocks != blocks) {
-    rc = mixed ? capture<true>(b, blocks, kLdsBytes) : capture<false>(b, blocks, kLdsBytes);
+  if (!b->exec || b->graph_mixed != mixed || b->graph_wide != wide ||
+      b->graph_blocks != blocks) {
+    rc = !mixed ? capture<false, false>(b, blocks, kLdsBytes)
+                : (wide ? capture<true, true>(b, blocks, kLdsBytes)
+                        : capture<true, false>(b, blocks, kLdsBytes));
     if (rc) return rc;
   }
   SDTFE_HIP(hipMemcpyAsync(b->d_items, b->h_items, sizeof(BatchItem) * n, hipMemcpyHostToDevice,

@@ -103,6 +103,16 @@ int check_params(const sdtfe_map_params *p) {
   return 0;
 }
 
+}  // namespace
+
+bool wide_records(int64_t n_tetra) {
+  if (n_tetra > kFaceRecNarrowTets) return true;
+  const char *e = std::getenv("SDTFE_WIDE_RECORDS");
+  return e && std::strcmp(e, "1") == 0;
+}
+
+namespace {
+
 int check_volume_params(const sdtfe_volume_params *p) {
   if (!p) return fail(SDTFE_EINVAL, "null volume parameters");
   if (p->grid_dim <= 0 || p->grid_dim > 65536) return fail(SDTFE_EINVAL, "grid_dim out of range");

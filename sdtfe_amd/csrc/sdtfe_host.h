@@ -120,6 +120,10 @@ void launch_volume_fp64(sdtfe_tess *t, const VolGeom &v, double *grid);
 // adapter.hip: the reference's tetra_data from Delaunay simplices
 int tetra_from_simplices(const double *particle_data, int n, const int *simplices, int nt,
                          int *tetra_data);
+// Whether a tessellation's face records need 64-bit offsets (more than
+// 2^25 tets); SDTFE_WIDE_RECORDS=1 takes the 64-bit kernels for every size
+// (they give the same maps; tests run both).
+bool wide_records(int64_t n_tetra);
 // field.hip
 int build_field(sdtfe_tess *t, const double *particle_data, const int *tetra_data,
                 Staging *st);

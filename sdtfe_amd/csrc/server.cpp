@@ -318,12 +318,6 @@ int sdtfe_server_submit(sdtfe_server *sv, const double *particle_data, int n_par
   if (p->grid_dim <= 0 || p->grid_dim > 65536 || p->n_mc_samp < 0 || !(p->box_len > 0) ||
       !(p->box_len_z > 0))
     return fail(SDTFE_EINVAL, "bad map parameters");
-  // checked here rather than at launch, where it would fail the launch's
-  // other requests too
-  if (p->precision == SDTFE_PREC_MIXED && (int64_t)n_tetra > kFaceRecNarrowTets)
-    return fail(SDTFE_EINVAL,
-                "tessellation too large for the batched mixed walk (2^25 tets); map it alone "
-                "or serve it in FP64");
   auto r = std::make_shared<Request>();
   r->pd.assign(particle_data, particle_data + 4 * (size_t)n_particles);
   r->tets.assign(tetra_data, tetra_data + 8 * (size_t)n_tetra);

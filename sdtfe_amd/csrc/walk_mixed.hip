@@ -313,7 +313,7 @@ void launch_first_pass_l(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc,
                          int *navg, int accumulate, const DeferQueue &dq) {
   constexpr int tw = 2 * wave_bw(L), th = 2 * wave_bh(L);
   const int nb = ((g.grid + tw - 1) / tw) * ((g.grid + th - 1) / th);
-  const bool wide = (int64_t)t->nt > kFaceRecNarrowTets;
+  const bool wide = wide_records(t->nt);
   if (mc.flt.full_slab && !wide)
     map_mixed_kernel<true, false, L><<<nb, 256, 0, t->stream>>>(
         t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
@@ -350,7 +350,7 @@ int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
   int rc = ensure_defer(t, want);
   if (rc) return rc;
   const MixedConsts mc = mixed_consts(t, g);
-  const bool wide = (int64_t)t->nt > kFaceRecNarrowTets;
+  const bool wide = wide_records(t->nt);
   for (;;) {
     const DeferQueue dq{t->defer_buf, t->defer_count, (int)std::min(t->defer_cap, npix)};
     defer_reset_kernel<<<1, 1, 0, t->stream>>>(t->defer_count);

@@ -1,6 +1,7 @@
 """Batched persistent launch: every item's map equals its standalone map
 (FP64: bit for bit, whatever the sample tiling), across uneven cutouts and
 repeated graph replays."""
+import os
 import sys
 
 import numpy as np
@@ -64,3 +65,34 @@ def test_autotuner_picks_a_candidate(cutouts, tmp_path):
     tuner.save()
     again = autotune.Autotuner(str(tmp_path / "tune.json"))
     assert again.table == tuner.table
+
+
+def test_wide_record_kernels_agree(cutouts):
+    """The kernels with 64-bit face-record offsets (taken for tessellations
+    beyond 2^25 tets; SDTFE_WIDE_RECORDS=1 takes them for any size) give the
+    same mixed maps as the 32-bit ones, single maps and batches alike."""
+    _say("[gpu] wide-record kernels vs narrow, single maps and a batch")
+    res = {}
+    for wide in ("", "1"):
+        os.environ["SDTFE_WIDE_RECORDS"] = wide
+        try:
+            alone = [t.map(G, S, seed=seed, precision=capi.PREC_MIXED)
+                     for t, G, S, seed in cutouts]
+            bufs = [capi.DeviceBuffer(G * G * 8) for _, G, _, _ in cutouts]
+            for buf in bufs:
+                buf.zero()
+            with capi.Batch() as b:
+                items = [(t, t.params(G, S, seed=seed, precision=capi.PREC_MIXED), buf)
+                         for (t, G, S, seed), buf in zip(cutouts, bufs)]
+                st, _ = b.run(items, lanes_per_pixel=[4] * len(items))
+            res[wide] = (alone, [buf.to_host() for buf in bufs], st)
+            for buf in bufs:
+                buf.close()
+        finally:
+            os.environ.pop("SDTFE_WIDE_RECORDS", None)
+    (a0, b0, s0), (a1, b1, s1) = res[""], res["1"]
+    for (m0, st0), (m1, st1), x0, x1, bs0, bs1 in zip(a0, a1, b0, b1, s0, s1):
+        np.testing.assert_array_equal(m1, m0)
+        np.testing.assert_array_equal(x1, x0)
+        np.testing.assert_array_equal(x1, m0)
+        assert st1["tet_steps"] == st0["tet_steps"] and bs1["tet_steps"] == bs0["tet_steps"]

@@ -78,22 +78,3 @@ def test_poisson_stream_reports_latency(requests):
         res = serve.run_stream(sv, requests, rate_hz=50.0, seed=1)
     assert res["p50_ms"] > 0 and res["p99_ms"] >= res["p50_ms"]
     assert res["sample_rays_per_s"] > 0
-
-
-def test_oversized_mixed_request_rejected_at_submit():
-    """A mixed server takes tessellations of up to 2^25 tets (32-bit face
-    record offsets in the batched walk): a larger one is refused at submit
-    (only its size is read), so it cannot fail a launch it shares."""
-    import ctypes as C
-    _say("[gpu] server refuses an oversized mixed request")
-    pd = np.zeros((4, 4))
-    tet = np.zeros((1, 8), dtype=np.int32)
-    rho = np.zeros(16)
-    with capi.Server(max_active=2, tile_budget=16, precision=capi.PREC_MIXED) as sv:
-        p = capi.Tessellation.params(None, 4, 1, precision=capi.PREC_MIXED)
-        rid = C.c_int64(-1)
-        rc = capi.lib().sdtfe_server_submit(sv.h, pd, 4, tet, (1 << 25) + 1, 1.0, C.byref(p),
-                                            rho, C.byref(rid))
-        assert rc == -1  # SDTFE_EINVAL
-        assert b"too large" in capi.lib().sdtfe_last_error()
-        assert sv.info()["queued"] == 0
