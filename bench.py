@@ -107,6 +107,13 @@ def measured_traffic(kernel, workload):
         return json.load(f).get(kernel, {}).get(workload)
 
 
+def _ordinal(k: int) -> str:
+    if k == 1:
+        return "single"  # every band
+    suffix = "th" if 10 <= k % 100 <= 20 else {1: "st", 2: "nd", 3: "rd"}.get(k % 10, "th")
+    return f"{k}{suffix}"
+
+
 def cpu_baseline(pos, tetra, cfg, seed, target_s, gpu_map):
     """Oracle (C, OpenMP) on every k-th 4-row band of the same map."""
     from oracle import oracle as O
@@ -139,7 +146,7 @@ def cpu_baseline(pos, tetra, cfg, seed, target_s, gpu_map):
     sel = sel.ravel()
     err = rel_err(gpu_map[sel], rho[sel]) if gpu_map is not None else np.zeros(1)
     return dict(value=rays / t, unit="sample-rays/s", cores=threads, kind="port",
-                sample=(f"every {k}th 4-row band of the {G}x{G} map ({rays} sample-rays, "
+                sample=(f"every {_ordinal(k)} 4-row band of the {G}x{G} map ({rays} sample-rays, "
                         f"{100.0 * rays / (G * G * S):.2f}% of the map), same tessellation "
                         f"and sample stream, {t:.1f} s"),
                 seconds=t, steps_per_ray=st["tet_steps"] / max(1, st["samples"] - st["missed"]),
