@@ -156,17 +156,33 @@ def test_dropin_compute_density(golden):
     np.testing.assert_array_equal(pd[:, 3], g[f"{name}_vertex_rho"])
     assert _rel(rho - 0.5, g[f"{name}_rho"]) <= 1e-9
     assert capi.lib().sdtfe_get_seed() == seed + 1
+    # SDTFE_PRECISION=mixed selects the mixed walk for the drop-in
+    import os
+    pd = to_particle_data(g["pos"])
+    mixed = np.zeros(G * G)
+    capi.set_seed(seed)
+    os.environ["SDTFE_PRECISION"] = "mixed"
+    try:
+        capi.compute_density(pd, g["tetra"], G, L, Lz, mixed, m, cen, S, ds)
+    finally:
+        os.environ.pop("SDTFE_PRECISION", None)
+    with capi.Tessellation(to_particle_data(g["pos"]), g["tetra"], p_mass=m) as t:
+        want, _ = t.map(G, S, box_len=L, box_len_z=Lz, center=cen, delta_sample=ds, seed=seed,
+                        precision=capi.PREC_MIXED)
+    np.testing.assert_array_equal(mixed, want)
+    assert _rel(mixed, g[f"{name}_rho"]) <= 1e-5
 
 
-def test_map_device_accumulates_and_repeats():
-    _say("[gpu] map_device accumulate + repeatability")
+@pytest.mark.parametrize("prec", [capi.PREC_FP64, capi.PREC_MIXED])
+def test_map_device_accumulates_and_repeats(prec):
+    _say(f"[gpu] map_device accumulate + repeatability, precision {prec}")
     pos = workloads.uniform(20000, 4)
     tetra = triangulate(pos)
     with capi.Tessellation(to_particle_data(pos), tetra) as t:
-        host, _ = t.map(128, 4, seed=9)
+        host, _ = t.map(128, 4, seed=9, precision=prec)
         buf = capi.DeviceBuffer(128 * 128 * 8)
         buf.zero()
-        p = t.params(128, 4, seed=9)
+        p = t.params(128, 4, seed=9, precision=prec)
         t.map_device(p, buf)
         t.map_device(p, buf)
         dev = buf.to_host()

@@ -75,11 +75,13 @@ def test_volume_perturbed_rays_match_oracle(oracle, golden, name, G, ss):
 
 @pytest.mark.parametrize("G,ss,L,cen", [(64, 2, 1.0, (0.5, 0.5, 0.5)),
                                         (45, 3, 0.6, (0.4, 0.55, 0.45)),
-                                        (1, 2, 1.0, (0.5, 0.5, 0.5))])
+                                        (1, 2, 1.0, (0.5, 0.5, 0.5)),
+                                        (96, 1, 1.1, (0.5, 0.5, 0.5))])
 def test_clustered_volume_matches_oracle(oracle, G, ss, L, cen):
     """131,072 clustered particles (halos): density grid against the
     oracle's compute_3d_density, a ragged grid (45 is no multiple of the
-    16 x 16 block) and a one-cell grid included."""
+    16 x 16 block), a one-cell grid and supersampling 1 (the column kernel,
+    pydtfe's default) on a box wider than the hull included."""
     _say(f"[gpu] clustered volume G={G} ss={ss}")
     pos = workloads.clustered(131072, seed=12)
     tetra = triangulate(pos)
