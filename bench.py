@@ -145,12 +145,43 @@ def cpu_baseline(pos, tetra, cfg, seed, target_s, gpu_map):
     sel[rows, :] = True
     sel = sel.ravel()
     err = rel_err(gpu_map[sel], rho[sel]) if gpu_map is not None else np.zeros(1)
-    return dict(value=rays / t, unit="sample-rays/s", cores=threads, kind="port",
+    ref = reference_baseline(pos, tetra, cfg, seed, target_s / 2, bands, args, threads)
+    return dict(value=rays / t, unit="sample-rays/s", cores=threads, kind="port", reference=ref,
                 sample=(f"every {_ordinal(k)} 4-row band of the {G}x{G} map ({rays} sample-rays, "
                         f"{100.0 * rays / (G * G * S):.2f}% of the map), same tessellation "
                         f"and sample stream, {t:.1f} s"),
                 seconds=t, steps_per_ray=st["tet_steps"] / max(1, st["samples"] - st["missed"]),
                 gpu_vs_cpu_max_rel_err=float(err.max()) if err.size else 0.0)
+
+
+def reference_baseline(pos, tetra, cfg, seed, target_s, bands, port_args, threads):
+    """The reference's own compiled pt_loc_2d and shoot_3d_ray (oracle/_ref,
+    built from the reference's sources; single-threaded, as its OpenMP loop
+    serialises on rand()) over every k-th 4-row band of the same map, and
+    the port over the same bands for a bit-for-bit comparison.  None when
+    the reference build is absent."""
+    from oracle import oracle as O
+    from sdtfe_amd.tessellate import to_particle_data
+    if not O.ref_available():
+        return None
+    G, S = cfg["grid"], cfg["samples"]
+    rs = O.RefSetup(to_particle_data(pos), tetra)
+    try:
+        geo = (G, cfg["box_len"], cfg["box_len_z"], cfg["center"], S, 0.0, seed)
+        t0 = time.perf_counter()
+        rs.map_bands(*geo, band_begin=bands // 2, band_step=bands)
+        t_band = max(time.perf_counter() - t0, 1e-6)
+        k = max(1, int(np.ceil(bands * t_band / target_s)))
+        t0 = time.perf_counter()
+        got, rays = rs.map_bands(*geo, band_begin=k // 2, band_step=k)
+        t = time.perf_counter() - t0
+    finally:
+        rs.close()
+    want, _, _ = O.map_bands(*port_args, band_begin=k // 2, band_step=k, n_threads=threads)
+    return dict(value=rays / t, unit="sample-rays/s", cores=1, kind="reference",
+                sample=(f"every {_ordinal(k)} 4-row band of the {G}x{G} map ({rays} sample-rays), "
+                        f"{t:.1f} s"),
+                equal_to_port=bool(np.array_equal(got, want)))
 
 
 def tessellation(name, pos, cache):
@@ -514,6 +545,10 @@ def main():
                            forms["fp64"]["map"] if "fp64" in forms else None)
         log(f"[bench] CPU {cpu['value']:.3e} sample-rays/s on {cpu['cores']} threads; "
             f"GPU/CPU {value / cpu['value']:.1f}x")
+        if cpu["reference"]:
+            r = cpu["reference"]
+            log(f"[bench] reference's own walk {r['value']:.3e} sample-rays/s on 1 thread "
+                f"({r['sample']}); equal to the port: {r['equal_to_port']}")
 
     out = {
         "metric": "sample-rays/s", "value": round(value, 1), "unit": "sample-rays/s",
@@ -545,6 +580,7 @@ def main():
     if cpu is not None:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
         out["fp64_vs_cpu_max_rel_err"] = cpu["gpu_vs_cpu_max_rel_err"]
+        out["cpu_reference"] = cpu["reference"]
     print(json.dumps(out), flush=True)
 
 

@@ -130,6 +130,10 @@ def ref():
         R.rayTetraPlucker_full.argtypes = [_dp, C.POINTER(C.c_void_p * 4), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), _dp, _dp, C.POINTER(C.c_int)]
         R.refh_rand_calls.restype = C.c_int64
+        R.refh_map_bands.restype = C.c_int64
+        R.refh_map_bands.argtypes = [_dp, _ip, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                     C.c_double, _dp, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int]
         R.interpolate_to_3d_grid.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double,
                                              _dp, C.c_double, C.c_double, C.c_double, C.c_uint]
         R.compute_3d_density.argtypes = [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double, _dp,
@@ -279,3 +283,40 @@ def volume_ray_xy(grid_dim, box_len, center, supersampling):
     x = i * dxy + off_x + is_ * ds + ds / 2.
     y = j * dxy + off_y + js * ds + ds / 2.
     return x, y
+
+
+class RefSetup:
+    """The reference's own field and triangle list of a tessellation
+    (pre_compute_vol, gradients, find_forward_hull, copy_fh_triangles from
+    the compiled reference), kept as C pointers for ref_map_bands."""
+
+    def __init__(self, pd, tetra, p_mass=1.0):
+        R = ref()
+        self.R = R
+        self.pd = np.ascontiguousarray(pd, dtype=np.float64)
+        self.tetra = np.ascontiguousarray(tetra, dtype=np.int32)
+        n, nt = self.pd.shape[0], self.tetra.shape[0]
+        ptr = C.c_void_p(self.pd.ctypes.data)
+        R.pre_compute_vol(C.byref(ptr), n, self.tetra, nt, p_mass)
+        self.nabla = R.gradients(self.tetra, nt, self.pd)
+        bm = R.create_bitmap(nt * 4)
+        n_tri = R.find_forward_hull(self.tetra, nt, self.pd, n, bm)
+        self.tri = R.copy_fh_triangles(self.tetra, nt, self.pd, n, n_tri, bm)
+        R.free(bm)
+
+    def map_bands(self, grid_dim, box_len, box_len_z, center, n_samp, delta_sample, seed,
+                  band_begin, band_step, rho=None):
+        """The reference's pt_loc_2d and shoot_3d_ray over every band_step-th
+        4-row band from band_begin (single-threaded); returns (rho, rays)."""
+        if rho is None:
+            rho = np.zeros(grid_dim * grid_dim)
+        rays = self.R.refh_map_bands(self.pd, self.tetra, self.nabla, self.tri, grid_dim,
+                                     box_len, box_len_z, rho, center[0], center[1], center[2],
+                                     n_samp, delta_sample, seed, band_begin, band_step)
+        return rho, int(rays)
+
+    def close(self):
+        if self.nabla:
+            self.R.free(C.c_void_p(self.nabla))
+            self.R.free(C.c_void_p(self.tri))
+            self.nabla = self.tri = None

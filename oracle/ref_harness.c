@@ -26,9 +26,12 @@
 
 #include "dtfe_oracle.h"
 
-/* The reference's own compute_density, in the same shared object. */
+/* The reference's own functions, in the same shared object. */
 extern void compute_density(double *, int, int *, int, int, double, double, double *, float,
                             double, double, double, const int, const double);
+extern double shoot_3d_ray(int, int *, double *, double *, double, double, double, double, double,
+                           int, int, int *);
+extern int pt_loc_2d(int *, double *, int *, double, double);
 
 static uintptr_t g_cd_lo, g_cd_hi;
 static uint64_t g_seed;
@@ -101,4 +104,52 @@ int rand(void) {
     return (int)sdo_draw31(g_seed, g_pixel, g_sample, coord);
   }
   return (int)sdo_draw31(g_seed, g_pixel, g_sample, g_draw++);
+}
+
+/* The pixels of the 4-row bands band_begin, band_begin + band_step, ... of
+ * a grid_dim^2 map, each as compute_density forms it (src/dtfe.c:613-645)
+ * with the reference's own pt_loc_2d and shoot_3d_ray, sample positions and
+ * perturbation corners from the shared stream; single-threaded.  For timing
+ * the reference's walk on a bounded part of a large map.  Returns the rays
+ * traced. */
+int64_t refh_map_bands(double *pd, int *tetra, double *nabla, int *tri, int grid_dim,
+                       double box_len, double box_len_z, double *rho, double cx, double cy,
+                       double cz, int n_samp, double delta_sample, uint64_t seed,
+                       int band_begin, int band_step) {
+  refh_init_range();
+  const double dxy = box_len / grid_dim;
+  const double off_x = -(box_len / 2.0) + cx, off_y = -(box_len / 2.0) + cy;
+  const double off_z = -(box_len_z / 2.0) + cz;
+  const double scale = delta_sample > dxy ? delta_sample : dxy;
+  const double soff = delta_sample > dxy ? (delta_sample - dxy) / 2.0 : 0.0;
+  int64_t rays = 0;
+  if (band_step < 1) band_step = 1;
+  for (int band = band_begin; band < grid_dim / 4; band += band_step)
+    for (int tx = 0; tx < grid_dim / 4; ++tx)
+      for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 4; ++k) {
+          const int px = tx * 4 + k, py = band * 4 + j;
+          const uint64_t pix = (uint64_t)py * (uint64_t)grid_dim + (uint64_t)px;
+          const double x = px * dxy + off_x - soff, y = py * dxy + off_y - soff;
+          double sum = 0.0;
+          int n_avg = n_samp;
+          for (int r = 0; r < n_samp; ++r) {
+            const double xt = x + ((double)sdo_draw31(seed, pix, (uint32_t)r, 0) / SDO_RAND_MAX) * scale;
+            const double yt = y + ((double)sdo_draw31(seed, pix, (uint32_t)r, 1) / SDO_RAND_MAX) * scale;
+            int start = 0;
+            const int ft = pt_loc_2d(tri, pd, &start, xt, yt);
+            ++rays;
+            if (ft == -1) {
+              --n_avg;
+              continue;
+            }
+            int crossed = 0;
+            refh_set_ray(seed, pix, (uint32_t)r);
+            sum += shoot_3d_ray(ft, tetra, nabla, pd, xt, yt, off_z, box_len_z, dxy, grid_dim,
+                                (int)pix, &crossed);
+            if (crossed == 0) --n_avg;
+          }
+          rho[pix] += n_avg > 0 ? sum / n_avg : 0.0;
+        }
+  return rays;
 }

@@ -139,3 +139,26 @@ def test_volume_perturbed_rays_match_reference(oracle, golden, name, G, ss):
                                      seed=7, n_threads=2)
     np.testing.assert_array_equal(grid, g[f"{name}_grid"])
     assert st["perturb_steps"] == int(g[f"{name}_rand_calls"]) > 100
+
+
+def test_reference_band_loop_matches_oracle(oracle):
+    """The reference's own pt_loc_2d and shoot_3d_ray driven over a band
+    sample of a map (the loop bench.py times as the reference's CPU path)
+    give the oracle's pixels bit for bit."""
+    if not oracle.ref_available():
+        pytest.skip("reference build not available")
+    pos = workloads.clustered(8192, seed=4)
+    tetra = triangulate(pos)
+    G, S, L, Lz, cen, ds, seed = 64, 5, 0.9, 0.7, (0.5, 0.5, 0.45), 0.02, 9
+    rs = oracle.RefSetup(to_particle_data(pos), tetra)
+    try:
+        got, rays = rs.map_bands(G, L, Lz, cen, S, ds, seed, band_begin=1, band_step=3)
+    finally:
+        rs.close()
+    pd = to_particle_data(pos)
+    oracle.density(pd, tetra, 1.0)
+    want, n, _ = oracle.map_bands(pd, tetra, oracle.gradients(tetra, pd),
+                                  oracle.triangles(tetra, pd), G, L, Lz, cen, S, ds, seed,
+                                  band_begin=1, band_step=3, n_threads=2)
+    assert rays == n == 16 * (G // 4) * len(range(1, G // 4, 3)) * S
+    np.testing.assert_array_equal(got, want)
