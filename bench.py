@@ -377,7 +377,7 @@ def bench_volume(a):
                 f"{ss} points per cell in z")
     kernel = ("volume_rays_kernel+volume_fold_kernel" if ss > 1 else "volume_fp64_kernel")
     tr = measured_traffic(kernel, workload)
-    cpu = None
+    cpu = cpu_ref = None
     if not a.no_cpu:
         from oracle import oracle as O
         if not os.path.exists(O.LIB_PATH):
@@ -396,6 +396,22 @@ def bench_volume(a):
                "sample": f"the whole {G}^3 grid (density, gradients, entry walks and rays), "
                          f"{t_cpu:.1f} s",
                "grid_bit_equal": same}
+        if O.ref_available():
+            # the reference's own code over the whole grid on one core
+            rs = O.RefSetup(to_particle_data(pos), tetra)
+            try:
+                t0 = time.perf_counter()
+                ref_grid, ref_rays = rs.volume(G, 1.0, (0.5, 0.5, 0.5), ss, a.seed)
+                t_ref = time.perf_counter() - t0
+            finally:
+                rs.close()
+            ref_same = bool(np.array_equal(grid, ref_grid))
+            log(f"[bench] reference's own evaluate_3d_ray_on_grid: {ref_rays / t_ref:.3e} rays/s "
+                f"on 1 core ({t_ref:.1f} s); GPU grid bit-equal: {ref_same}")
+            cpu_ref = {"value": ref_rays / t_ref, "unit": "rays/s", "cores": 1,
+                       "kind": "reference",
+                       "sample": f"the whole {G}^3 grid, rays only ({t_ref:.1f} s)",
+                       "grid_bit_equal": ref_same}
     out = {"metric": "rays/s", "value": round(rays / (ms / 1e3), 1), "unit": "rays/s",
            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -411,6 +427,7 @@ def bench_volume(a):
                         "tet_steps_per_launch": st["tet_steps"], "cell_visits": cell_visits,
                         "kernel_ms": round(kernel_ms, 3)},
            "cpu_baseline": cpu,
+           "cpu_reference": cpu_ref,
            "stats": {k: st[k] for k in ("samples", "missed", "failed", "tet_steps",
                                         "perturb_steps")}}
     if tr is not None:

@@ -130,6 +130,9 @@ def ref():
         R.rayTetraPlucker_full.argtypes = [_dp, C.POINTER(C.c_void_p * 4), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), _dp, _dp, C.POINTER(C.c_int)]
         R.refh_rand_calls.restype = C.c_int64
+        R.refh_volume.restype = C.c_int64
+        R.refh_volume.argtypes = [_dp, _ip, C.c_void_p, C.c_void_p, C.c_int, C.c_double, _dp,
+                                  C.c_double, C.c_double, C.c_double, C.c_uint]
         R.refh_map_bands.restype = C.c_int64
         R.refh_map_bands.argtypes = [_dp, _ip, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                                      C.c_double, _dp, C.c_double, C.c_double, C.c_double,
@@ -314,6 +317,17 @@ class RefSetup:
                                      box_len, box_len_z, rho, center[0], center[1], center[2],
                                      n_samp, delta_sample, seed, band_begin, band_step)
         return rho, int(rays)
+
+    def volume(self, grid_dim, box_len, center, supersampling, seed, grid=None):
+        """interpolate_to_3d_grid's grid by the reference's own
+        evaluate_3d_ray_on_grid (single-threaded; perturbation corners keyed
+        as in the oracle); returns (grid, rays)."""
+        if grid is None:
+            grid = np.zeros(grid_dim ** 3)
+        self.R.refh_set_ray(seed, 0, 0)  # names the stream's seed
+        rays = self.R.refh_volume(self.pd, self.tetra, self.nabla, self.tri, grid_dim, box_len,
+                                  grid, center[0], center[1], center[2], supersampling)
+        return grid, int(rays)
 
     def close(self):
         if self.nabla:

@@ -32,6 +32,8 @@ extern void compute_density(double *, int, int *, int, int, double, double, doub
 extern double shoot_3d_ray(int, int *, double *, double *, double, double, double, double, double,
                            int, int, int *);
 extern int pt_loc_2d(int *, double *, int *, double, double);
+extern void evaluate_3d_ray_on_grid(int, int *, double *, double *, double, double, double, double,
+                                    int, double, int *, unsigned, double *);
 
 static uintptr_t g_cd_lo, g_cd_hi;
 static uint64_t g_seed;
@@ -151,5 +153,38 @@ int64_t refh_map_bands(double *pd, int *tetra, double *nabla, int *tri, int grid
           }
           rho[pix] += n_avg > 0 ? sum / n_avg : 0.0;
         }
+  return rays;
+}
+
+/* interpolate_to_3d_grid's column loop (src/dtfe.c:686-731) over the
+ * reference's own pt_loc_2d and evaluate_3d_ray_on_grid, single-threaded,
+ * with each ray's perturbation corners keyed (seed, column, is * ss + js)
+ * so that the grid can be compared with other builds bit for bit.  grid
+ * (grid_dim^3) is accumulated into.  Returns the rays traced. */
+int64_t refh_volume(double *pd, int *tetra, double *nabla, int *tri, int grid_dim,
+                    double box_len, double *grid, double cx, double cy, double cz,
+                    unsigned ss) {
+  refh_init_range();
+  const double dxy = box_len / grid_dim;
+  const double off_x = -(box_len / 2.0) + cx, off_y = -(box_len / 2.0) + cy;
+  const double off_z = -(box_len / 2.0) + cz;
+  const double ds = dxy / ss;
+  int64_t rays = 0;
+  const uint64_t seed = g_seed;
+  for (int i = 0; i < grid_dim; ++i)
+    for (int j = 0; j < grid_dim; ++j) {
+      const size_t index0 = ((size_t)i * grid_dim + j) * grid_dim;
+      for (int is = 0; is < (int)ss; ++is)
+        for (int js = 0; js < (int)ss; ++js) {
+          const double x = i * dxy + off_x + is * ds + ds / 2.;
+          const double y = j * dxy + off_y + js * ds + ds / 2.;
+          int start = 0, crossed = 0;
+          const int ft = pt_loc_2d(tri, pd, &start, x, y);
+          refh_set_ray(seed, (uint64_t)i * grid_dim + j, (uint32_t)(is * (int)ss + js));
+          evaluate_3d_ray_on_grid(ft, tetra, nabla, pd, x, y, off_z, box_len, grid_dim, dxy,
+                                  &crossed, ss, grid + index0);
+          ++rays;
+        }
+    }
   return rays;
 }

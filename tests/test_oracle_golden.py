@@ -162,3 +162,19 @@ def test_reference_band_loop_matches_oracle(oracle):
                                   band_begin=1, band_step=3, n_threads=2)
     assert rays == n == 16 * (G // 4) * len(range(1, G // 4, 3)) * S
     np.testing.assert_array_equal(got, want)
+
+
+def test_reference_volume_loop_matches_fixture(oracle, golden):
+    """The harness's column loop over the reference's own
+    evaluate_3d_ray_on_grid (what bench.py times for the volumetric grid)
+    reproduces G6's perturbed-ray grid."""
+    if not oracle.ref_available():
+        pytest.skip("reference build not available")
+    g = golden("g6_volume.npz")
+    rs = oracle.RefSetup(to_particle_data(g["crafted_pos"]), g["crafted_tetra"])
+    try:
+        grid, rays = rs.volume(8, 1.0, (0.5, 0.5, 0.5), 2, 7)
+    finally:
+        rs.close()
+    assert rays == 8 * 8 * 4
+    np.testing.assert_array_equal(grid, g["crafted8_grid"])
