@@ -145,7 +145,7 @@ def cpu_baseline(pos, tetra, cfg, seed, target_s, gpu_map):
     sel[rows, :] = True
     sel = sel.ravel()
     err = rel_err(gpu_map[sel], rho[sel]) if gpu_map is not None else np.zeros(1)
-    ref = reference_baseline(pos, tetra, cfg, seed, target_s / 2, bands, args, threads)
+    ref = reference_baseline(pos, tetra, cfg, seed, target_s / 2, bands, args, threads, gpu_map)
     return dict(value=rays / t, unit="sample-rays/s", cores=threads, kind="port", reference=ref,
                 sample=(f"every {_ordinal(k)} 4-row band of the {G}x{G} map ({rays} sample-rays, "
                         f"{100.0 * rays / (G * G * S):.2f}% of the map), same tessellation "
@@ -154,12 +154,12 @@ def cpu_baseline(pos, tetra, cfg, seed, target_s, gpu_map):
                 gpu_vs_cpu_max_rel_err=float(err.max()) if err.size else 0.0)
 
 
-def reference_baseline(pos, tetra, cfg, seed, target_s, bands, port_args, threads):
+def reference_baseline(pos, tetra, cfg, seed, target_s, bands, port_args, threads, gpu_map):
     """The reference's own compiled pt_loc_2d and shoot_3d_ray (oracle/_ref,
     built from the reference's sources; single-threaded, as its OpenMP loop
     serialises on rand()) over every k-th 4-row band of the same map, and
-    the port over the same bands for a bit-for-bit comparison.  None when
-    the reference build is absent."""
+    the port over the same bands, for bit-for-bit comparisons with the port
+    and with the GPU's FP64 map.  None when the reference build is absent."""
     from oracle import oracle as O
     from sdtfe_amd.tessellate import to_particle_data
     if not O.ref_available():
@@ -178,10 +178,16 @@ def reference_baseline(pos, tetra, cfg, seed, target_s, bands, port_args, thread
     finally:
         rs.close()
     want, _, _ = O.map_bands(*port_args, band_begin=k // 2, band_step=k, n_threads=threads)
+    rows = np.zeros(G, dtype=bool)
+    for b in range(k // 2, bands, k):
+        rows[4 * b:4 * b + 4] = True
+    sel = np.repeat(rows, G)
     return dict(value=rays / t, unit="sample-rays/s", cores=1, kind="reference",
                 sample=(f"every {_ordinal(k)} 4-row band of the {G}x{G} map ({rays} sample-rays), "
                         f"{t:.1f} s"),
-                equal_to_port=bool(np.array_equal(got, want)))
+                equal_to_port=bool(np.array_equal(got, want)),
+                equal_to_gpu_fp64=(None if gpu_map is None
+                                   else bool(np.array_equal(got[sel], gpu_map[sel]))))
 
 
 def tessellation(name, pos, cache):
@@ -565,7 +571,8 @@ def main():
         if cpu["reference"]:
             r = cpu["reference"]
             log(f"[bench] reference's own walk {r['value']:.3e} sample-rays/s on 1 thread "
-                f"({r['sample']}); equal to the port: {r['equal_to_port']}")
+                f"({r['sample']}); equal to the port: {r['equal_to_port']}, to the GPU's FP64 "
+                f"map: {r['equal_to_gpu_fp64']}")
 
     out = {
         "metric": "sample-rays/s", "value": round(value, 1), "unit": "sample-rays/s",
