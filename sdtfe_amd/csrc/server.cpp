@@ -50,6 +50,7 @@ struct Request {
   Clock::time_point t_submit, t_admit, t_start, t_done;
   bool started = false, done = false;
   int status = 0;
+  std::string error;  // why status is not 0 (the failing thread's message)
   sdtfe_stats stats{};
 };
 
@@ -103,7 +104,9 @@ void add_stats(sdtfe_stats &a, const sdtfe_stats &s) {
 }
 
 void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status) {
-  // caller holds no lock; device work of r is complete
+  // caller holds no lock; device work of r is complete.  A failing status
+  // comes with this thread's last error, which the waiter cannot see.
+  std::string error = status ? std::string(sdtfe_last_error()) : std::string();
   if (status == 0 && r->d_map) {
     // on the request's own stream: a copy on the null stream would wait for
     // every other request's setup
@@ -111,9 +114,10 @@ void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status) {
     std::vector<double> m(npix);
     if (hipMemcpyAsync(m.data(), r->d_map, sizeof(double) * npix, hipMemcpyDeviceToHost,
                        r->tess->stream) != hipSuccess ||
-        hipStreamSynchronize(r->tess->stream) != hipSuccess)
+        hipStreamSynchronize(r->tess->stream) != hipSuccess) {
       status = SDTFE_EHIP;
-    else
+      error = "copying the map back failed";
+    } else
       for (size_t i = 0; i < npix; ++i) r->out[i] += m[i];
   }
   if (r->d_map) dev_free(r->d_map, r->tess->stream);
@@ -124,6 +128,7 @@ void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status) {
   std::vector<int>().swap(r->tets);
   std::lock_guard<std::mutex> g(sv->mu);
   r->status = status;
+  r->error = std::move(error);
   r->done = true;
   r->t_done = Clock::now();
   sv->used_bytes -= r->footprint;
@@ -356,7 +361,9 @@ int sdtfe_server_wait(sdtfe_server *sv, int64_t id, sdtfe_request_info *info) {
     info->footprint_bytes = r->footprint;
     info->stats = r->stats;
   }
-  return r->status;
+  if (r->status)
+    return fail(r->status, "request " + std::to_string(id) + " failed: " + r->error);
+  return 0;
 }
 
 int sdtfe_server_get_info(sdtfe_server *sv, sdtfe_server_info *info) {

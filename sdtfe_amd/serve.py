@@ -64,29 +64,43 @@ def run_stream(server: capi.Server, reqs: list[Request], rate_hz: float | None,
     gaps = rng.exponential(1.0 / rate_hz, len(reqs)) if rate_hz else np.zeros(len(reqs))
     ids: list[int] = []
     results: dict[int, dict] = {}
+    errors: list[str] = []
     done = threading.Event()
 
     def waiter():
-        k = 0
-        while k < len(reqs):
-            while k >= len(ids):
-                time.sleep(0.0005)
-            _, info = server.wait(ids[k])
-            results[k] = info
-            k += 1
-        done.set()
+        # waits for every request, also past a failed one, so the server drains
+        try:
+            k = 0
+            while k < len(reqs):
+                while k >= len(ids) and not errors:
+                    time.sleep(0.0005)
+                if k >= len(ids):
+                    break  # submission stopped on an error
+                try:
+                    _, info = server.wait(ids[k])
+                    results[k] = info
+                except capi.SdtfeError as e:
+                    errors.append(str(e))
+                k += 1
+        finally:
+            done.set()
 
     th = threading.Thread(target=waiter, daemon=True)
     t0 = time.perf_counter()
     th.start()
     t_next = t0
-    for r, gap in zip(reqs, gaps):
-        t_next += gap
-        dt = t_next - time.perf_counter()
-        if dt > 0:
-            time.sleep(dt)
-        ids.append(server.submit(r.pd, r.tetra, params_for(server, r)))
+    try:
+        for r, gap in zip(reqs, gaps):
+            t_next += gap
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(server.submit(r.pd, r.tetra, params_for(server, r)))
+    except capi.SdtfeError as e:
+        errors.append(str(e))
     done.wait()
+    if errors:
+        raise RuntimeError(f"{len(errors)} request(s) failed; first: {errors[0]}")
     wall = time.perf_counter() - t0
     lat = np.array([results[k]["latency_ms"] for k in range(len(reqs))])
     rays = sum(r.rays for r in reqs)

@@ -78,3 +78,22 @@ def test_poisson_stream_reports_latency(requests):
         res = serve.run_stream(sv, requests, rate_hz=50.0, seed=1)
     assert res["p50_ms"] > 0 and res["p99_ms"] >= res["p50_ms"]
     assert res["sample_rays_per_s"] > 0
+
+
+def test_failed_request_reports_its_error(requests):
+    """A request whose tetra_data names a vertex past the particles fails at
+    set-up (the index check, before any walk); its waiter gets the message,
+    and the requests around it still complete."""
+    _say("[gpu] server: a failing request among good ones")
+    good = requests[:3]
+    bad_tets = good[1].tetra.copy()
+    bad_tets[5, 2] = good[1].pd.shape[0] + 3
+    with capi.Server(max_active=4, tile_budget=64) as sv:
+        ids = [sv.submit(good[0].pd, good[0].tetra, serve.params_for(sv, good[0])),
+               sv.submit(good[1].pd, bad_tets, serve.params_for(sv, good[1])),
+               sv.submit(good[2].pd, good[2].tetra, serve.params_for(sv, good[2]))]
+        _, i0 = sv.wait(ids[0])
+        with pytest.raises(capi.SdtfeError, match="out of range"):
+            sv.wait(ids[1])
+        _, i2 = sv.wait(ids[2])
+    assert i0["status"] == 0 and i2["status"] == 0
