@@ -175,7 +175,7 @@ void setup_loop(sdtfe_server *sv) {
     if (rc == 0 && (hipMemsetAsync(r->d_map, 0, sizeof(double) * npix, r->tess->stream) !=
                         hipSuccess ||
                     hipStreamSynchronize(r->tess->stream) != hipSuccess))
-      rc = SDTFE_EHIP;
+      rc = fail(SDTFE_EHIP, "clearing the map buffer failed");
     if (rc == 0) {
       r->lpp = sv->cfg.lanes_per_pixel > 0 ? sv->cfg.lanes_per_pixel
                                            : map_lanes(r->params.n_mc_samp);
@@ -256,7 +256,7 @@ void walk_loop(sdtfe_server *sv) {
                               : batch_run_ranges(sv->batch, (int)ts.size(), ts.data(), ps.data(),
                                                  outs.data(), lp.data(), b0.data(), cnt.data(),
                                                  st.data(), &tot);
-    std::vector<std::shared_ptr<Request>> finished;
+    std::vector<std::shared_ptr<Request>> finished, empty;  // empty: had no tiles
     {
       std::lock_guard<std::mutex> g(sv->mu);
       sv->launches++;
@@ -272,7 +272,7 @@ void walk_loop(sdtfe_server *sv) {
       // requests with no tiles at all
       for (auto it = sv->active.begin(); it != sv->active.end();) {
         if ((*it)->n_tiles == 0) {
-          finished.push_back(*it);
+          empty.push_back(*it);
           it = sv->active.erase(it);
         } else {
           ++it;
@@ -280,6 +280,7 @@ void walk_loop(sdtfe_server *sv) {
       }
     }
     for (auto &f : finished) finish(sv, f, rc);
+    for (auto &f : empty) finish(sv, f, 0);  // in no launch: the launch's status is not theirs
   }
 }
 
