@@ -28,3 +28,19 @@ def test_binding_loads_without_gpu():
     capi.lib()
     assert os.path.exists(capi.LIB_PATH)
     assert capi.lib().sdtfe_last_error() is not None
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without the built library the binding (and pydtfe above it) raises; there is no CPU
+    fallback."""
+    import numpy as np
+    import pytest
+
+    from sdtfe_amd import pydtfe
+    monkeypatch.setattr(capi, "LIB_PATH", str(tmp_path / "libsdtfe_amd.so"))
+    monkeypatch.setattr(capi, "_lib", None)
+    with pytest.raises(capi.SdtfeError, match="not built"):
+        capi.lib()
+    pos = np.random.default_rng(0).random((64, 3))
+    with pytest.raises(capi.SdtfeError, match="not built"):
+        pydtfe.compute_surface_density(pos, [0.5, 0.5, 0.5], 8, 1.0, 1.0, 1.0, 1.0, 0.0, 1)
