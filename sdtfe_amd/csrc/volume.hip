@@ -121,6 +121,11 @@ struct CellSink {
     double &cell = win.w[iz & (kWinCells - 1)];
     cell = cell + t;
   }
+  // fine points e..l-1 of one tet, term(k) each, upward
+  template <class F>
+  SDTFE_HD void run(int e, int l, const F &term) {
+    for (int k = e; k < l; ++k) add(k, term(k));
+  }
   SDTFE_HD void finish() {
     if (win.base >= 0) win.store(col, n, pairs);
   }
@@ -133,59 +138,56 @@ constexpr uint64_t kNoTerm = 0x7FF4A5A5A5A5A5A5ull;
 // Ray kernel's sink: term for fine point k goes to slot k of the ray's row;
 // slots between two fine points of the ray that no tet holds get kNoTerm,
 // and [first, last] records the ray's slots.  With pairs (16-byte aligned
-// rows) slots go out as aligned pairs (2m, 2m + 1) held in registers until
-// the ray moves past them, half the stores of slot-by-slot writes.  Fine
-// points that do not come strictly upward set bad (the row is then not
-// used).
+// rows, nzs even) slots go out as aligned pairs (2m, 2m + 1): an even slot
+// is held in p0 until its odd partner is written, half the stores of
+// slot-by-slot writes.  A tet whose fine points do not start above the
+// ray's last slot sets bad (the row is then not used).  Within a tet the
+// slots are consecutive, so the checks run once per tet and the loop over
+// its fine points only computes and places terms.
 struct TermSink {
   double *row;
   bool pairs;
   int first = -1, last = -1;
   bool bad = false;
-  int pend = -1;  // pair held: slots pend, pend + 1
-  double p0 = 0.0, p1 = 0.0;
+  double p0 = __builtin_bit_cast(double, kNoTerm);  // pairs: even slot last, if held
 
+  // slot k, one past the previous slot put (or the ray's first)
   SDTFE_HD void put(int k, double t) {
     if (!pairs) {
       row[k] = t;
-      return;
-    }
-    const int pr = k & ~1;
-    if (pr != pend) {
-      flush();
-      pend = pr;
-      p0 = p1 = __builtin_bit_cast(double, kNoTerm);
-    }
-    if (k & 1)
-      p1 = t;
-    else
+    } else if (k & 1) {
+      *reinterpret_cast<double2 *>(row + (k - 1)) = make_double2(p0, t);
+      p0 = __builtin_bit_cast(double, kNoTerm);
+    } else {
       p0 = t;
+    }
   }
-  SDTFE_HD void flush() {
-    if (pend >= 0) *reinterpret_cast<double2 *>(row + pend) = make_double2(p0, p1);
-    pend = -1;
-  }
-  SDTFE_HD void add(int k, double t) {
+  // fine points e..l-1 (l > e) of one tet, term(k) each
+  template <class F>
+  SDTFE_HD void run(int e, int l, const F &term) {
     if (bad) return;
-    if (k <= last) {
+    if (e <= last) {
       bad = true;
       return;
     }
     if (first < 0)
-      first = k;
+      first = e;
     else
-      for (int g = last + 1; g < k; ++g) put(g, __builtin_bit_cast(double, kNoTerm));
-    put(k, t);
-    last = k;
+      for (int g = last + 1; g < e; ++g) put(g, __builtin_bit_cast(double, kNoTerm));
+    for (int k = e; k < l; ++k) put(k, term(k));
+    last = l - 1;
   }
   SDTFE_HD void finish() {
-    if (!bad) flush();
+    // an even last slot is still held: its pair goes out with an empty partner
+    if (pairs && !bad && first >= 0 && (last & 1) == 0)
+      *reinterpret_cast<double2 *>(row + last) =
+          make_double2(p0, __builtin_bit_cast(double, kNoTerm));
   }
 };
 
 // evaluate_3d_ray_on_grid (src/dtfe.c:376-516): the walk of shoot_3d_ray
-// with its perturbation; in each tet crossed, each fine point's term
-// ss_weight * (linear field at the point) goes to sink.add in walk order.
+// with its perturbation; in each tet crossed, the fine points' terms
+// ss_weight * (linear field at the point) go to sink.run in walk order.
 // A failed walk keeps what it added, as the reference does.  Returns the
 // tets crossed (0 on failure).
 template <class Sink>
@@ -232,10 +234,11 @@ SDTFE_HD inline int ray_walk64(const Rec64 *__restrict__ rec, int entry, double 
     fine_range(v, h.z_in, h.z_out, e, l);
     if (l > e) {
       const double fixed = r.rho_a + r.gx * (ox - r.ax) + r.gy * (oy - r.ay) - r.gz * r.az;
-      for (int k = e; k < l; ++k) {
+      const double gz = r.gz;
+      sink.run(e, l, [&](int k) {
         const double z = v.off_z + v.dzs * (double)k + v.dzs / 2.;
-        sink.add(k, (fixed + r.gz * z) * v.ss_weight);
-      }
+        return (fixed + gz * z) * v.ss_weight;
+      });
     }
   }
   sink.finish();
