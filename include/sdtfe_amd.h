@@ -11,6 +11,16 @@
  *   gradients         replaces src/triangulate.h:27 (src/triangulate.c:233-260)
  *   interpolate_to_3d_grid  replaces src/dtfe.h:17-18 (src/dtfe.c:667-735)
  *   compute_3d_density      replaces src/dtfe.h:20-21 (src/dtfe.c:657-665)
+ *   find_forward_hull replaces src/triangulate.h:28 (src/triangulate.c:263-316)
+ *   copy_fh_triangles replaces src/triangulate.h:29 (src/triangulate.c:318-395)
+ *   convert_to_qhdata, rotate3d, shuffle_buff
+ *                     replace src/utils.h:10-12 (src/utils.c:8-79)
+ *   set_bit, clear_bit, get_bit, create_bitmap, delete_bitmap
+ *                     replace src/bitmap.h:9-17 (src/bitmap.c:8-31)
+ *   write_image, write_rho  replace src/io.h:35-36 (src/io.c:13-95)
+ *
+ * The reference's triangulate (src/triangulate.h:25, Qhull) is not
+ * replaced: Qhull stays on the host.
  *
  * Behaviour kept from the reference: particle_data is N x 4 float64 AoS
  * (x, y, z, w) and column 3 is overwritten with the vertex density;
@@ -80,6 +90,46 @@ void compute_3d_density(double *particle_data, int n_particles, int *tetra_data,
                         double center_x, double center_y, double center_z,
                         unsigned supersampling);
 
+/* src/triangulate.h:28: marks the hull faces whose outward normal points
+ * to -z (where vertical rays enter the hull) in bitmap, bit 4 t + f for
+ * face f of tet t in the reference's bitmap layout (bit f of word t), and
+ * returns their count.  Computed on the GPU. */
+int find_forward_hull(int *tetra_data, int n_tetra, double *particle_data, int n_particles,
+                      uint32_t *bitmap);
+
+/* src/triangulate.h:29: a malloc'd n_tri x 8 int array (free() it) with one
+ * row (a, b, c, n_CB, n_AC, n_BA, tet, 0) per face bitmap marks, in tet
+ * then face order; n_* is the row across each edge or -1. */
+int *copy_fh_triangles(int *tetra_data, int n_tetra, double *particle_data, int n_particles,
+                       int n_tri, uint32_t *bitmap);
+
+/* src/utils.h:10: planar float32 positions (n_data x values, then y, then
+ * z) to a malloc'd n_data x 4 float64 array (x, y, z, 0); free() it. */
+double *convert_to_qhdata(const float *rm_buff, size_t n_data);
+/* src/utils.h:11: rotates rows 0 .. n-1 about center by
+ * Qx(theta[0]) Qy(theta[1]) Qz(theta[2]). */
+void rotate3d(double *data, size_t n_particles, double theta[3], double center[3]);
+/* src/utils.h:12: partial Fisher-Yates shuffle of the first n_shuffle rows
+ * (N x 4) drawing from the C library's rand(), as the reference does. */
+void shuffle_buff(double *data, int n_particles_in_buff, int n_shuffle);
+
+/* src/bitmap.h:9-17: bitmaps of uint32_t words holding 4 bits each (the
+ * reference's layout): bit n is bit n % 4 of word n / 4.  create_bitmap(n)
+ * returns n / 4 + 1 zeroed words (free with delete_bitmap). */
+void set_bit(uint32_t *words, int n);
+void clear_bit(uint32_t *words, int n);
+int get_bit(uint32_t *words, int n);
+uint32_t *create_bitmap(int n);
+void delete_bitmap(uint32_t *words);
+
+/* src/io.h:35: the map as an 8-bit palette TIFF (log-scaled indices 0-128,
+ * Jet palette), one strip, uncompressed; no file if a pixel is negative or
+ * NaN. */
+void write_image(char *file_name, double *rho, int grid_dim);
+/* src/io.h:36: the raw map, rho_size float64 values; returns the count
+ * written. */
+int write_rho(char *fname, double **rho, int rho_size);
+
 /* ---------------------------------------------------------- extended */
 
 enum {
@@ -144,6 +194,13 @@ int sdtfe_tetra_from_simplices(const double *particle_data, int n_particles,
                                const int *simplices, int n_tetra, int *tetra_data);
 
 const char *sdtfe_last_error(void);
+/* Status of the calling thread's last call of a void or pointer-returning
+ * drop-in above (compute_density, pre_compute_vol, gradients,
+ * interpolate_to_3d_grid, compute_3d_density, find_forward_hull,
+ * copy_fh_triangles): 0 on success, else its SDTFE_E* code, with
+ * sdtfe_last_error() describing it.  The reference's prototypes carry no
+ * status; they print the failure to stderr as well. */
+int sdtfe_dropin_status(void);
 void sdtfe_set_seed(uint64_t seed);
 uint64_t sdtfe_get_seed(void);
 

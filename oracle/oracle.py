@@ -223,14 +223,26 @@ def compute_density(pd, tetra, grid_dim, box_len, box_len_z, p_mass, center, n_s
 
 
 def map_bands(pd, tetra, nabla, tri, grid_dim, box_len, box_len_z, center, n_samp,
-              delta_sample=0.0, seed=0, band_begin=0, band_step=1, n_threads=0, rho=None):
+              delta_sample=0.0, seed=0, band_begin=0, band_step=1, n_threads=0, rho=None,
+              n_avg=None):
+    """Pixels of every band_step-th 4-row band from band_begin; n_avg (int32,
+    grid_dim^2, optional) receives those pixels' sample counts."""
     if rho is None:
         rho = np.zeros(grid_dim * grid_dim)
     st = Stats()
     n = lib().sdo_map_bands(pd, tetra, nabla, tri, grid_dim, box_len, box_len_z, rho,
                             center[0], center[1], center[2], n_samp, delta_sample, seed,
-                            band_begin, band_step, n_threads, C.byref(st), None)
+                            band_begin, band_step, n_threads, C.byref(st),
+                            n_avg.ctypes.data if n_avg is not None else None)
     return rho, int(n), st.as_dict()
+
+
+def band_rows(grid_dim, band_begin, band_step):
+    """Boolean mask (grid_dim^2) of the pixels map_bands computes."""
+    rows = np.zeros(grid_dim, dtype=bool)
+    for b in range(band_begin, grid_dim // 4, band_step):
+        rows[4 * b:4 * b + 4] = True
+    return np.repeat(rows, grid_dim)
 
 
 def volume(pd, tetra, grid_dim, box_len, center, supersampling, seed=0, p_mass=None,

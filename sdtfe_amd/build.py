@@ -33,6 +33,7 @@ UNITS = {
     "batch.hip": ["-ffp-contract=off"],
     "volume.hip": ["-ffp-contract=off"],
     "adapter.hip": ["-ffp-contract=off"],
+    "utils_io.cpp": ["-ffp-contract=off"],
     "capi.cpp": ["-x", "hip", "-ffp-contract=off"],
     "server.cpp": ["-x", "hip", "-ffp-contract=off"],
 }

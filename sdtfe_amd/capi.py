@@ -72,6 +72,7 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _vp = C.c_void_p
 
 _SIGS = {
@@ -122,6 +123,19 @@ _SIGS = {
                                       C.c_double, C.c_double, C.c_double, C.c_uint]),
     "compute_3d_density": (None, [_dp, C.c_int, _ip, C.c_int, C.c_int, C.c_double, _dp,
                                   C.c_float, C.c_double, C.c_double, C.c_double, C.c_uint]),
+    "sdtfe_dropin_status": (C.c_int, []),
+    "find_forward_hull": (C.c_int, [_ip, C.c_int, _dp, C.c_int, _u32p]),
+    "copy_fh_triangles": (_vp, [_ip, C.c_int, _dp, C.c_int, C.c_int, _u32p]),
+    "convert_to_qhdata": (_vp, [_f32p, C.c_size_t]),
+    "rotate3d": (None, [_dp, C.c_size_t, _dp, _dp]),
+    "shuffle_buff": (None, [_dp, C.c_int, C.c_int]),
+    "set_bit": (None, [_u32p, C.c_int]),
+    "clear_bit": (None, [_u32p, C.c_int]),
+    "get_bit": (C.c_int, [_u32p, C.c_int]),
+    "create_bitmap": (_vp, [C.c_int]),
+    "delete_bitmap": (None, [_vp]),
+    "write_image": (None, [C.c_char_p, _dp, C.c_int]),
+    "write_rho": (C.c_int, [C.c_char_p, C.POINTER(_vp), C.c_int]),
 }
 
 _lib = None
@@ -153,6 +167,14 @@ def header_functions() -> list[str]:
 def check(rc: int) -> None:
     if rc != 0:
         raise SdtfeError(f"sdtfe error {rc}: {lib().sdtfe_last_error().decode()}")
+
+
+def check_dropin(name: str) -> None:
+    """Raise SdtfeError if the calling thread's last drop-in call failed (the
+    reference's void prototypes carry no status; the library keeps it)."""
+    rc = int(lib().sdtfe_dropin_status())
+    if rc != 0:
+        raise SdtfeError(f"{name} failed ({rc}): {lib().sdtfe_last_error().decode()}")
 
 
 def device_count() -> int:
@@ -450,6 +472,7 @@ def compute_density(particle_data, tetra_data, grid_dim, box_len, box_len_z, rho
     lib().compute_density(particle_data, particle_data.shape[0], tet, tet.shape[0], grid_dim,
                           box_len, box_len_z, rho, p_mass, center[0], center[1], center[2],
                           n_mc_samp, delta_sample)
+    check_dropin("compute_density")
 
 
 def interpolate_to_3d_grid(particle_data, tetra_data, grid_dim, box_len, grid, center,
@@ -462,6 +485,7 @@ def interpolate_to_3d_grid(particle_data, tetra_data, grid_dim, box_len, grid, c
     lib().interpolate_to_3d_grid(particle_data, particle_data.shape[0], tet, tet.shape[0],
                                  grid_dim, box_len, grid, center[0], center[1], center[2],
                                  supersampling)
+    check_dropin("interpolate_to_3d_grid")
 
 
 def compute_3d_density(particle_data, tetra_data, grid_dim, box_len, rho, p_mass, center,
@@ -474,6 +498,7 @@ def compute_3d_density(particle_data, tetra_data, grid_dim, box_len, rho, p_mass
     lib().compute_3d_density(particle_data, particle_data.shape[0], tet, tet.shape[0],
                              grid_dim, box_len, rho, p_mass, center[0], center[1], center[2],
                              supersampling)
+    check_dropin("compute_3d_density")
 
 
 def tetra_from_simplices(particle_data, simplices) -> np.ndarray:
@@ -492,6 +517,86 @@ def tetra_from_simplices(particle_data, simplices) -> np.ndarray:
     out = np.empty((simp.shape[0], 8), dtype=np.int32)
     check(lib().sdtfe_tetra_from_simplices(p, p.shape[0], simp, simp.shape[0], out))
     return out
+
+
+def _libc():
+    return C.CDLL(None)
+
+
+def _take_malloced(ptr, count, ctype, dtype):
+    """Copy a malloc'd C array into numpy and free it."""
+    if not ptr:
+        raise SdtfeError(f"allocation failed: {lib().sdtfe_last_error().decode()}")
+    try:
+        return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,)).astype(
+            dtype, copy=True)
+    finally:
+        _libc().free(C.c_void_p(ptr))
+
+
+def find_forward_hull(tetra_data, particle_data, bitmap=None):
+    """find_forward_hull (src/triangulate.h:28) through the ABI: returns
+    (count, bitmap), bitmap in the reference's layout (bit f of word t for
+    face f of tet t; n_tetra + 1 uint32 words)."""
+    tet = _c64(tetra_data, np.int32)
+    pd = _c64(particle_data)
+    if bitmap is None:
+        bitmap = np.zeros(tet.shape[0] + 1, dtype=np.uint32)
+    n = lib().find_forward_hull(tet, tet.shape[0], pd, pd.shape[0], bitmap)
+    check_dropin("find_forward_hull")
+    return int(n), bitmap
+
+
+def copy_fh_triangles(tetra_data, particle_data, n_tri, bitmap) -> np.ndarray:
+    """copy_fh_triangles (src/triangulate.h:29) through the ABI: n_tri x 8
+    int32 rows (a, b, c, n_CB, n_AC, n_BA, tet, 0)."""
+    tet = _c64(tetra_data, np.int32)
+    pd = _c64(particle_data)
+    bm = np.ascontiguousarray(bitmap, dtype=np.uint32)
+    ptr = lib().copy_fh_triangles(tet, tet.shape[0], pd, pd.shape[0], int(n_tri), bm)
+    rows = _take_malloced(ptr, 8 * max(int(n_tri), 1), C.c_int, np.int32)
+    check_dropin("copy_fh_triangles")
+    return rows[:8 * int(n_tri)].reshape(int(n_tri), 8)
+
+
+def convert_to_qhdata(planar_xyz) -> np.ndarray:
+    """convert_to_qhdata (src/utils.h:10): planar float32 (x..., y..., z...)
+    to N x 4 float64 rows."""
+    buf = np.ascontiguousarray(planar_xyz, dtype=np.float32).ravel()
+    n = buf.size // 3
+    ptr = lib().convert_to_qhdata(buf, n)
+    return _take_malloced(ptr, 4 * max(n, 1), C.c_double, np.float64)[:4 * n].reshape(n, 4)
+
+
+def rotate3d(data: np.ndarray, n: int, theta, center) -> None:
+    """rotate3d (src/utils.h:11) in place on rows 0..n-1 of N x 4 float64."""
+    if data.dtype != np.float64 or not data.flags.c_contiguous or data.shape[1] != 4:
+        raise ValueError("data must be C-contiguous N x 4 float64")
+    lib().rotate3d(data, int(n), _c64(theta), _c64(center))
+
+
+def shuffle_buff(data: np.ndarray, n_in_buff: int, n_shuffle: int) -> None:
+    """shuffle_buff (src/utils.h:12) in place, drawing from the C library's
+    rand() (seed it with libc srand)."""
+    if data.dtype != np.float64 or not data.flags.c_contiguous or data.shape[1] != 4:
+        raise ValueError("data must be C-contiguous N x 4 float64")
+    lib().shuffle_buff(data, int(n_in_buff), int(n_shuffle))
+
+
+def write_image(path: str, rho) -> None:
+    """write_image (src/io.h:35): the map as an 8-bit palette TIFF."""
+    r = _c64(rho).ravel()
+    g = int(round(r.size ** 0.5))
+    if g * g != r.size:
+        raise ValueError("rho must hold grid_dim^2 values")
+    lib().write_image(os.fsencode(path), r, g)
+
+
+def write_rho(path: str, rho) -> int:
+    """write_rho (src/io.h:36): the raw float64 map; returns the count written."""
+    r = _c64(rho).ravel()
+    ptr = C.c_void_p(r.ctypes.data)
+    return int(lib().write_rho(os.fsencode(path), C.byref(ptr), r.size))
 
 
 def set_seed(seed: int) -> None:

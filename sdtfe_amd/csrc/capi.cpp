@@ -17,6 +17,9 @@
 namespace sdtfe {
 
 static thread_local std::string g_err;
+static thread_local int g_dropin_status = 0;
+
+void set_dropin_status(int rc) { g_dropin_status = rc; }
 
 int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -248,6 +251,39 @@ int tess_create_staged(const double *particle_data, int n, const int *tetra_data
   return create(particle_data, n, tetra_data, nt, p_mass, false, st, out);
 }
 
+void link_hull_rows(int *rows, int f) {
+  // Neighbours across shared edges (src/triangulate.c:358-392): slot 5 for
+  // edge (a, b), 4 for (a, c), 3 for (b, c); the reference compares all
+  // pairs, so of several triangles on one edge the highest-numbered other
+  // one is kept.  Here the edges are sorted by their vertex pair instead.
+  struct Edge {
+    int lo, hi, tri, slot;
+  };
+  std::vector<Edge> e;
+  e.reserve(3 * (size_t)f);
+  static const int pair[3][3] = {{0, 1, 5}, {0, 2, 4}, {1, 2, 3}};
+  for (int k = 0; k < f; ++k)
+    for (const auto &q : pair) {
+      const int u = rows[8 * (size_t)k + q[0]], v = rows[8 * (size_t)k + q[1]];
+      e.push_back({std::min(u, v), std::max(u, v), k, q[2]});
+    }
+  std::sort(e.begin(), e.end(), [](const Edge &x, const Edge &y) {
+    return x.lo != y.lo ? x.lo < y.lo : (x.hi != y.hi ? x.hi < y.hi : x.tri < y.tri);
+  });
+  for (size_t s0 = 0; s0 < e.size();) {
+    size_t s1 = s0;
+    while (s1 < e.size() && e[s1].lo == e[s0].lo && e[s1].hi == e[s0].hi) ++s1;
+    // sorted by tri within the group: the highest other is the last, or
+    // the one before it for the last itself
+    for (size_t a = s0; a < s1; ++a) {
+      const int best = e[s1 - 1].tri != e[a].tri ? e[s1 - 1].tri
+                                                  : (s1 - s0 > 1 ? e[s1 - 2].tri : -1);
+      if (best >= 0 && best != e[a].tri) rows[8 * (size_t)e[a].tri + e[a].slot] = best;
+    }
+    s0 = s1;
+  }
+}
+
 namespace {
 
 void free_tess(sdtfe_tess *t) {
@@ -335,6 +371,7 @@ using namespace sdtfe;
 extern "C" {
 
 const char *sdtfe_last_error(void) { return g_err.c_str(); }
+int sdtfe_dropin_status(void) { return g_dropin_status; }
 void sdtfe_set_seed(uint64_t seed) { g_seed.store(seed); }
 uint64_t sdtfe_get_seed(void) { return g_seed.load(); }
 
@@ -402,36 +439,7 @@ int sdtfe_tess_triangles(const sdtfe_tess *t, int *rows, int capacity) {
     r[6] = tet[k];
     r[7] = 0;
   }
-  // Neighbours across shared edges (src/triangulate.c:358-392): slot 5 for
-  // edge (a, b), 4 for (a, c), 3 for (b, c); the reference compares all
-  // pairs, so of several triangles on one edge the highest-numbered other
-  // one is kept.  Here the edges are sorted by their vertex pair instead.
-  struct Edge {
-    int lo, hi, tri, slot;
-  };
-  std::vector<Edge> e;
-  e.reserve(3 * (size_t)f);
-  static const int pair[3][3] = {{0, 1, 5}, {0, 2, 4}, {1, 2, 3}};
-  for (int k = 0; k < f; ++k)
-    for (const auto &q : pair) {
-      const int u = rows[8 * (size_t)k + q[0]], v = rows[8 * (size_t)k + q[1]];
-      e.push_back({std::min(u, v), std::max(u, v), k, q[2]});
-    }
-  std::sort(e.begin(), e.end(), [](const Edge &x, const Edge &y) {
-    return x.lo != y.lo ? x.lo < y.lo : (x.hi != y.hi ? x.hi < y.hi : x.tri < y.tri);
-  });
-  for (size_t s0 = 0; s0 < e.size();) {
-    size_t s1 = s0;
-    while (s1 < e.size() && e[s1].lo == e[s0].lo && e[s1].hi == e[s0].hi) ++s1;
-    // sorted by tri within the group: the highest other is the last, or
-    // the one before it for the last itself
-    for (size_t a = s0; a < s1; ++a) {
-      const int best = e[s1 - 1].tri != e[a].tri ? e[s1 - 1].tri
-                                                  : (s1 - s0 > 1 ? e[s1 - 2].tri : -1);
-      if (best >= 0 && best != e[a].tri) rows[8 * (size_t)e[a].tri + e[a].slot] = best;
-    }
-    s0 = s1;
-  }
+  link_hull_rows(rows, f);
   return 0;
 }
 
@@ -686,6 +694,7 @@ void compute_density(double *particle_data, int n_particles, int *tetra_data, in
         for (int i = 0; i < n_particles; ++i) particle_data[4 * (size_t)i + 3] = r[i];
     }
   }
+  set_dropin_status(rc);
   if (rc) std::fprintf(stderr, "compute_density (sdtfe_amd): %s\n", g_err.c_str());
   sdtfe_tess_destroy(t);
 }
@@ -701,6 +710,7 @@ void pre_compute_vol(double **particle_data, int n_particles, int *tetra_data, i
     if (rc == 0)
       for (int i = 0; i < n_particles; ++i) (*particle_data)[4 * (size_t)i + 3] = r[i];
   }
+  set_dropin_status(rc);
   if (rc) std::fprintf(stderr, "pre_compute_vol (sdtfe_amd): %s\n", g_err.c_str());
   sdtfe_tess_destroy(t);
 }
@@ -728,6 +738,7 @@ void interpolate_to_3d_grid(double *particle_data, int n_particles, int *tetra_d
   if (rc == 0)
     rc = dropin_volume(t, grid_dim, box_len, grid, center_x, center_y, center_z,
                        supersampling);
+  set_dropin_status(rc);
   if (rc) std::fprintf(stderr, "interpolate_to_3d_grid (sdtfe_amd): %s\n", g_err.c_str());
   sdtfe_tess_destroy(t);
 }
@@ -746,13 +757,17 @@ void compute_3d_density(double *particle_data, int n_particles, int *tetra_data,
   }
   if (rc == 0)
     rc = dropin_volume(t, grid_dim, box_len, rho, center_x, center_y, center_z, supersampling);
+  set_dropin_status(rc);
   if (rc) std::fprintf(stderr, "compute_3d_density (sdtfe_amd): %s\n", g_err.c_str());
   sdtfe_tess_destroy(t);
 }
 
 double *gradients(int *tetra_data, int n_tetra, double *particle_data) {
   double *nabla = (double *)std::malloc(sizeof(double) * 3 * (size_t)(n_tetra > 0 ? n_tetra : 1));
-  if (!nabla) return nullptr;
+  if (!nabla) {
+    set_dropin_status(fail(SDTFE_ENOMEM, "gradients: host allocation failed"));
+    return nullptr;
+  }
   // vertex count: the largest index the tets name
   int n = 0;
   for (int64_t i = 0; i < 4 * (int64_t)n_tetra; ++i) {
@@ -762,6 +777,7 @@ double *gradients(int *tetra_data, int n_tetra, double *particle_data) {
   sdtfe_tess *t = nullptr;
   int rc = sdtfe_tess_create_field(particle_data, n, tetra_data, n_tetra, &t);
   if (rc == 0) rc = sdtfe_tess_gradients(t, nabla);
+  set_dropin_status(rc);
   if (rc) {
     std::fprintf(stderr, "gradients (sdtfe_amd): %s\n", g_err.c_str());
     std::free(nabla);
@@ -769,6 +785,74 @@ double *gradients(int *tetra_data, int n_tetra, double *particle_data) {
   }
   sdtfe_tess_destroy(t);
   return nabla;
+}
+
+// src/triangulate.h:28 (src/triangulate.c:263-316): sets bit 4 t + f of
+// bitmap (bitmap.h layout: bit f of word t) for every hull face f of tet t
+// (neighbour -1) whose (t1 x t2)_z < 0, and returns their count; the test
+// runs on the GPU.  Bits already set stay set, as set_bit leaves them.
+int find_forward_hull(int *tetra_data, int n_tetra, double *particle_data, int n_particles,
+                      uint32_t *bitmap) {
+  int rc = 0, count = 0;
+  if (n_tetra < 0 || n_particles < 0 || (n_tetra > 0 && (!tetra_data || !bitmap)) ||
+      (n_particles > 0 && !particle_data))
+    rc = fail(SDTFE_EINVAL, "find_forward_hull: bad argument");
+  std::vector<unsigned char> mask(n_tetra > 0 ? (size_t)n_tetra : 0);
+  if (!rc) rc = hull_mask_host(particle_data, n_particles, tetra_data, n_tetra, mask.data());
+  if (!rc)
+    for (int t = 0; t < n_tetra; ++t) {
+      bitmap[t] |= mask[t];
+      count += __builtin_popcount(mask[t]);
+    }
+  set_dropin_status(rc);
+  if (rc) std::fprintf(stderr, "find_forward_hull (sdtfe_amd): %s\n", g_err.c_str());
+  return count;
+}
+
+// src/triangulate.h:29 (src/triangulate.c:318-395): a malloc'd n_tri x 8
+// array (free() it) of the faces bitmap marks, in tet then face order:
+// (a, b, c, n_CB, n_AC, n_BA, tet, 0) with the face's vertices in the
+// reference's order per face and the triangle across each edge (-1 where
+// none).  The rows are emitted on the GPU and linked on the host.  A bitmap
+// marking more than n_tri faces is an error (the reference writes past its
+// array); rows beyond the marked faces are zero.
+int *copy_fh_triangles(int *tetra_data, int n_tetra, double *particle_data, int n_particles,
+                       int n_tri, uint32_t *bitmap) {
+  int rc = 0;
+  int *rows = (int *)std::calloc(8 * (size_t)(n_tri > 0 ? n_tri : 1), sizeof(int));
+  if (!rows) rc = fail(SDTFE_ENOMEM, "copy_fh_triangles: host allocation failed");
+  if (!rc && (n_tetra < 0 || n_particles < 0 || n_tri < 0 ||
+              (n_tetra > 0 && (!tetra_data || !bitmap)) || (n_particles > 0 && !particle_data)))
+    rc = fail(SDTFE_EINVAL, "copy_fh_triangles: bad argument");
+  int marked = 0;
+  std::vector<unsigned char> mask(!rc && n_tetra > 0 ? (size_t)n_tetra : 0);
+  if (!rc)
+    for (int t = 0; t < n_tetra; ++t) {
+      mask[t] = (unsigned char)(bitmap[t] & 15u);
+      marked += __builtin_popcount(mask[t]);
+    }
+  if (!rc && marked > n_tri)
+    rc = fail(SDTFE_EINVAL, "copy_fh_triangles: the bitmap marks more than n_tri faces");
+  if (!rc && marked > 0) {
+    std::vector<int> abc(3 * (size_t)marked), tet((size_t)marked);
+    rc = hull_rows_host(particle_data, n_particles, tetra_data, n_tetra, mask.data(), abc.data(),
+                        tet.data(), marked);
+    if (!rc) {
+      for (int k = 0; k < marked; ++k) {
+        int *r = rows + 8 * (size_t)k;
+        r[0] = abc[3 * (size_t)k];
+        r[1] = abc[3 * (size_t)k + 1];
+        r[2] = abc[3 * (size_t)k + 2];
+        r[3] = r[4] = r[5] = -1;
+        r[6] = tet[k];
+        r[7] = 0;
+      }
+      link_hull_rows(rows, marked);
+    }
+  }
+  set_dropin_status(rc);
+  if (rc) std::fprintf(stderr, "copy_fh_triangles (sdtfe_amd): %s\n", g_err.c_str());
+  return rows;
 }
 
 }  // extern "C"

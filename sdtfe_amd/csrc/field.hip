@@ -575,4 +575,100 @@ int build_entry_grid(sdtfe_tess *t) {
   return 0;
 }
 
+// ------------------------------------------- forward hull of host arrays
+// find_forward_hull and copy_fh_triangles take host arrays and no
+// tessellation handle: the tets and particles go up on a stream of their
+// own, the same kernels as a tessellation's setup run, and the results come
+// back.  Indices are checked first (SDTFE_ETOPO).
+namespace {
+
+struct HostHull {
+  hipStream_t s = nullptr;
+  double *pos = nullptr;
+  int *tets = nullptr;
+  ~HostHull() {
+    if (!s) return;
+    dev_free(pos, s);
+    dev_free(tets, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
+  int init(const double *pd, int n, const int *tetra, int nt) {
+    SDTFE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    if (dev_malloc((void **)&pos, sizeof(double) * 4 * (size_t)(n > 0 ? n : 1), s) !=
+            hipSuccess ||
+        dev_malloc((void **)&tets, sizeof(int) * 8 * (size_t)(nt > 0 ? nt : 1), s) != hipSuccess)
+      return fail(SDTFE_ENOMEM, "forward hull: device allocation failed");
+    if (n > 0)
+      SDTFE_HIP(hipMemcpyAsync(pos, pd, sizeof(double) * 4 * (size_t)n, hipMemcpyHostToDevice, s));
+    if (nt > 0)
+      SDTFE_HIP(
+          hipMemcpyAsync(tets, tetra, sizeof(int) * 8 * (size_t)nt, hipMemcpyHostToDevice, s));
+    Scratch sc(s);
+    int *bad = nullptr;
+    SDTFE_RC(sc.get(&bad, 1));
+    SDTFE_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+    if (nt > 0) validate_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(tets, nt, n, bad);
+    int bad_h = 0;
+    SDTFE_HIP(hipMemcpyAsync(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    SDTFE_HIP(hipStreamSynchronize(s));
+    if (bad_h) return fail(SDTFE_ETOPO, "tetra_data holds an index out of range");
+    return 0;
+  }
+};
+
+}  // namespace
+
+int hull_mask_host(const double *pd, int n, const int *tetra, int nt, unsigned char *mask) {
+  if (nt <= 0) return 0;
+  HostHull h;
+  SDTFE_RC(h.init(pd, n, tetra, nt));
+  Scratch sc(h.s);
+  unsigned char *dmask = nullptr;
+  int *cnt = nullptr;
+  SDTFE_RC(sc.get(&dmask, (size_t)nt));
+  SDTFE_RC(sc.get(&cnt, (size_t)nt));
+  hull_mask_kernel<<<blocks_for(nt, 256), 256, 0, h.s>>>(reinterpret_cast<const double4 *>(h.pos),
+                                                         h.tets, nt, dmask, cnt);
+  SDTFE_HIP(hipGetLastError());
+  SDTFE_HIP(hipMemcpyAsync(mask, dmask, (size_t)nt, hipMemcpyDeviceToHost, h.s));
+  SDTFE_HIP(hipStreamSynchronize(h.s));
+  return 0;
+}
+
+int hull_rows_host(const double *pd, int n, const int *tetra, int nt, const unsigned char *mask,
+                   int *abc, int *tet, int n_faces) {
+  if (nt <= 0 || n_faces <= 0) return 0;
+  // each tet's first row: the count of marked faces before it
+  std::vector<int> off_h((size_t)nt);
+  int64_t total = 0;
+  for (int t = 0; t < nt; ++t) {
+    off_h[t] = (int)total;
+    total += __builtin_popcount(mask[t] & 15u);
+  }
+  if (total != n_faces) return fail(SDTFE_EINVAL, "forward hull: face count mismatch");
+  HostHull h;
+  SDTFE_RC(h.init(pd, n, tetra, nt));
+  Scratch sc(h.s);
+  unsigned char *dmask = nullptr;
+  int *off = nullptr, *dtet = nullptr, *dabc = nullptr;
+  double *dxy = nullptr;
+  SDTFE_RC(sc.get(&dmask, (size_t)nt));
+  SDTFE_RC(sc.get(&off, (size_t)nt));
+  SDTFE_RC(sc.get(&dtet, (size_t)n_faces));
+  SDTFE_RC(sc.get(&dabc, 3 * (size_t)n_faces));
+  SDTFE_RC(sc.get(&dxy, 6 * (size_t)n_faces));
+  SDTFE_HIP(hipMemcpyAsync(dmask, mask, (size_t)nt, hipMemcpyHostToDevice, h.s));
+  SDTFE_HIP(hipMemcpyAsync(off, off_h.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice,
+                           h.s));
+  emit_tri_kernel<<<blocks_for(nt, 256), 256, 0, h.s>>>(reinterpret_cast<const double4 *>(h.pos),
+                                                        h.tets, nt, dmask, off, dxy, dtet, dabc);
+  SDTFE_HIP(hipGetLastError());
+  SDTFE_HIP(hipMemcpyAsync(abc, dabc, sizeof(int) * 3 * (size_t)n_faces, hipMemcpyDeviceToHost,
+                           h.s));
+  SDTFE_HIP(hipMemcpyAsync(tet, dtet, sizeof(int) * (size_t)n_faces, hipMemcpyDeviceToHost, h.s));
+  SDTFE_HIP(hipStreamSynchronize(h.s));
+  return 0;
+}
+
 }  // namespace sdtfe

@@ -128,4 +128,17 @@ bool wide_records(int64_t n_tetra);
 int build_field(sdtfe_tess *t, const double *particle_data, const int *tetra_data,
                 Staging *st);
 int build_entry_grid(sdtfe_tess *t);
+// Forward-hull faces of host arrays, on the GPU (no tessellation handle):
+// mask[t] bit f marks face f of tet t (src/triangulate.c:274-311); the
+// faces a mask marks as vertex triples and tets in tet, then face order
+// (src/triangulate.c:325-355), n_faces being the mask's count.
+int hull_mask_host(const double *pd, int n, const int *tetra, int nt, unsigned char *mask);
+int hull_rows_host(const double *pd, int n, const int *tetra, int nt, const unsigned char *mask,
+                   int *abc, int *tet, int n_faces);
+// Neighbour slots 3-5 of forward-hull triangle rows (a, b, c, n_CB, n_AC,
+// n_BA, tet, 0) across shared edges, as copy_fh_triangles links them
+// (src/triangulate.c:358-392).
+void link_hull_rows(int *rows, int n_rows);
+// The status of the calling thread's last drop-in call (0: success).
+void set_dropin_status(int rc);
 }  // namespace sdtfe

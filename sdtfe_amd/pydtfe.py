@@ -36,6 +36,9 @@ import numpy as np
 from . import capi
 from .tessellate import triangulate
 
+__all__ = ["compute_surface_density", "compute_3d_density", "interpolate_field2grid",
+           "compute_particle_density"]
+
 _SHUFFLE_PIXEL = (1 << 63) + 0x5348  # stream key reserved for the shuffle
 
 
@@ -212,4 +215,5 @@ def compute_particle_density(particle_pos, particle_mass):
     ptr = C.c_void_p(pd.ctypes.data)
     capi.lib().pre_compute_vol(C.byref(ptr), pd.shape[0], tetra, tetra.shape[0],
                                float(np.float32(particle_mass)))
+    capi.check_dropin("pre_compute_vol")
     return pd[:, 3].astype(np.float32)

@@ -57,14 +57,16 @@ def params_for(server: capi.Server, req: Request) -> capi.MapParams:
 
 
 def run_stream(server: capi.Server, reqs: list[Request], rate_hz: float | None,
-               seed: int = 0) -> dict:
+               seed: int = 0, maps: list | None = None) -> dict:
     """Submit reqs at Poisson arrival times (rate_hz; None = all at once),
-    wait for all, and summarise latencies and throughput."""
+    wait for all, and summarise latencies and throughput.  maps (a list,
+    optional) receives each request's map in request order."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_hz, len(reqs)) if rate_hz else np.zeros(len(reqs))
     ids: list[int] = []
     results: dict[int, dict] = {}
     errors: list[str] = []
+    got: dict[int, np.ndarray] = {}
     done = threading.Event()
 
     def waiter():
@@ -77,8 +79,10 @@ def run_stream(server: capi.Server, reqs: list[Request], rate_hz: float | None,
                 if k >= len(ids):
                     break  # submission stopped on an error
                 try:
-                    _, info = server.wait(ids[k])
+                    rho, info = server.wait(ids[k])
                     results[k] = info
+                    if maps is not None:
+                        got[k] = rho
                 except capi.SdtfeError as e:
                     errors.append(str(e))
                 k += 1
@@ -102,6 +106,8 @@ def run_stream(server: capi.Server, reqs: list[Request], rate_hz: float | None,
     if errors:
         raise RuntimeError(f"{len(errors)} request(s) failed; first: {errors[0]}")
     wall = time.perf_counter() - t0
+    if maps is not None:
+        maps.extend(got[k] for k in range(len(reqs)))
     lat = np.array([results[k]["latency_ms"] for k in range(len(reqs))])
     rays = sum(r.rays for r in reqs)
     return dict(requests=len(reqs), wall_s=wall, rate_hz=rate_hz,

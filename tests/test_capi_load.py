@@ -44,3 +44,33 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     pos = np.random.default_rng(0).random((64, 3))
     with pytest.raises(capi.SdtfeError, match="not built"):
         pydtfe.compute_surface_density(pos, [0.5, 0.5, 0.5], 8, 1.0, 1.0, 1.0, 1.0, 0.0, 1)
+
+
+def test_c_caller_links_against_library_alone(tmp_path):
+    """A C program declaring the reference's prototypes (compute_density,
+    pre_compute_vol, gradients, find_forward_hull, copy_fh_triangles, the
+    bitmap and write_rho) links against libsdtfe_amd.so with no other object:
+    every symbol it needs is exported."""
+    import subprocess
+    libdir = os.path.dirname(capi.LIB_PATH)
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c_caller.c")
+    exe = str(tmp_path / "c_caller")
+    r = subprocess.run(["gcc", "-O2", "-std=c99", "-Wall", "-Werror", src, "-o", exe,
+                        "-L" + libdir, "-lsdtfe_amd", "-Wl,-rpath," + libdir,
+                        "-Wl,--no-undefined"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_dropin_failure_raises_without_device():
+    """With no HIP device the void drop-in fails and the binding raises
+    (nothing falls back to a CPU path)."""
+    import numpy as np
+    import pytest
+    if capi.device_count() > 0:
+        pytest.skip("a HIP device is present (the GPU tests cover failures there)")
+    pd = np.random.default_rng(0).random((8, 4))
+    tet = np.zeros((1, 8), dtype=np.int32)
+    tet[0, :4] = [0, 1, 2, 3]
+    tet[0, 4:] = -1
+    with pytest.raises(capi.SdtfeError, match="no HIP device"):
+        capi.compute_density(pd, tet, 8, 1.0, 1.0, np.zeros(64), 1.0, (0.5, 0.5, 0.5), 1, 0.0)
