@@ -10,10 +10,16 @@ through the tets, and pixel values are accumulated in HBM.  Default workload:
 2048x2048 pixels, 16 samples per pixel.  Particles are synthetic; the
 tessellation (host Qhull) and the upload are outside the timed region.
 
-Prints one JSON line: sample-rays/s of the whole job (value), the walk
-kernel's roofline (algorithmic bytes per tet crossed / kernel time from HIP
-events on the library's stream), and the CPU oracle timed on the host cores
-over a bounded subset of the same map (every k-th 4-row band).
+Prints one JSON line: sample-rays/s of the whole job (value); the walk
+kernel's roofline (physical HBM bytes per launch from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE profile, stamped with the kernel's sources, over the
+kernel time from HIP events on the library's stream in this run; the
+algorithmic rate at 88 B per tet crossed beside it); a serving leg (a
+Poisson stream of 256 cutout requests at 150 per second through the
+continuous-batching server, both precisions: p50/p99 latency, capacity and
+the mixed maps' error against the FP64 maps); and the CPU oracle timed on the
+host cores over a bounded subset of the same map (every k-th 4-row band),
+with the CPU model.
 """
 from __future__ import annotations
 
@@ -95,16 +101,53 @@ def time_form(tess, cfg, precision, steps, warmup, seed):
                 stats=st_last, map=acc)
 
 
+# Sources that decide a walk kernel's memory traffic: a profile taken from
+# other sources no longer describes the kernel that runs.
+KERNEL_SOURCES = {
+    "map_mixed_kernel": ["walk_mixed.hip", "walk_mixed_core.h", "walk_defer.h", "walk_common.h",
+                         "walk_fp64_core.h", "sdtfe_internal.h", "field.hip"],
+    "map_fp64_kernel": ["walk_fp64.hip", "walk_fp64_core.h", "walk_common.h",
+                        "sdtfe_internal.h", "field.hip"],
+}
+
+
+def source_stamp(kernel):
+    """SHA-256 (first 16 hex digits) over the kernel's source files."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES.get(kernel, []):
+        with open(os.path.join(ROOT, "sdtfe_amd", "csrc", name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), each
-    scaled by the same counter's reading on a copy of known size; None when
-    that workload was not profiled."""
+    scaled by the same counter's reading on a copy of known size, with the
+    source stamp the profile was taken at; None when that workload was not
+    profiled.  'stale' is set when the kernel's sources have changed since."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get(kernel, {}).get(workload)
+        tr = json.load(f).get(kernel, {}).get(workload)
+    if tr is not None:
+        tr = dict(tr)
+        tr["stale"] = tr.get("source_stamp") != source_stamp(kernel)
+    return tr
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or None
 
 
 def _ordinal(k: int) -> str:
@@ -330,6 +373,47 @@ def bench_serve(a):
     print(json.dumps(out), flush=True)
 
 
+SERVE_RATE_HZ = 150.0  # arrivals per second of the serving leg in the default run
+
+
+def serving_leg(n_requests, seed, rate_hz=SERVE_RATE_HZ):
+    """Poisson stream of n_requests cutout requests at rate_hz through the
+    continuous-batching server, once per precision: capacity (all requests
+    at once), p50/p99 latency from submission to the map on the host, and
+    the mixed maps' largest relative error against the FP64 maps."""
+    from sdtfe_amd import capi, serve
+    t0 = time.perf_counter()
+    reqs = serve.make_requests(n_requests, seed=seed, workers=16)
+    log(f"[bench] serving: {len(reqs)} requests tessellated in {time.perf_counter() - t0:.1f} s")
+    rays = sum(r.rays for r in reqs)
+    out = {"workload": f"Poisson stream of {len(reqs)} cutout requests (1e3-2e5 particles, "
+                       f"64-1024 px, 4-16 samples/pixel) at {rate_hz:g}/s",
+           "requests": len(reqs), "sample_rays": rays, "rate_hz": rate_hz}
+    maps = {}
+    for name, prec in (("fp64", capi.PREC_FP64), ("mixed", capi.PREC_MIXED)):
+        kw = dict(hbm_budget_bytes=64 << 30, max_active=64, tile_budget=8192, precision=prec)
+        with capi.Server(**kw) as srv:
+            cap = serve.run_stream(srv, reqs, None, seed=seed)
+        m = []
+        with capi.Server(**kw) as srv:
+            res = serve.run_stream(srv, reqs, rate_hz, seed=seed + 1, maps=m)
+        maps[name] = m
+        out[name] = {"p50_ms": round(res["p50_ms"], 2), "p99_ms": round(res["p99_ms"], 2),
+                     "mean_ms": round(res["mean_ms"], 2),
+                     "sample_rays_per_s": round(res["sample_rays_per_s"], 1),
+                     "capacity_maps_per_s": round(cap["requests"] / cap["wall_s"], 1),
+                     "capacity_sample_rays_per_s": round(cap["sample_rays_per_s"], 1)}
+        log(f"[bench] serving {name}: p50 {res['p50_ms']:.1f} ms, p99 {res['p99_ms']:.1f} ms, "
+            f"capacity {cap['requests'] / cap['wall_s']:.1f} maps/s")
+    worst = 0.0
+    for a_, b_ in zip(maps["mixed"], maps["fp64"]):
+        e = rel_err(a_, b_)
+        if e.size:
+            worst = max(worst, float(e.max()))
+    out["mixed"]["max_rel_err_vs_fp64"] = worst
+    return out
+
+
 def bench_volume(a):
     """Volumetric grid (compute_3d_density's interpolation, src/dtfe.c:
     667-735) of the 1,048,576 clustered particles' density: a 512^3 grid of
@@ -443,6 +527,33 @@ def bench_volume(a):
     print(json.dumps(out), flush=True)
 
 
+def roofline_entry(kernel, workload, f):
+    """The walk kernel against HBM: physical bytes per launch (the committed
+    FETCH_SIZE / WRITE_SIZE profile of this kernel and workload) over the
+    kernel's time measured in this run with HIP events.  Beside it the
+    algorithmic rate (88 B per tet crossed, which counts no reuse between
+    rays and so exceeds the HBM peak when rays share records through the
+    caches) and the unit the profile finds busiest."""
+    algo = f["tet_steps"] * BYTES_PER_STEP / (f["kernel_ms"] / 1e3) / 1e9
+    tr = measured_traffic(kernel, workload)
+    phys = None if tr is None else tr["bytes_per_launch"] / (f["kernel_ms"] / 1e3) / 1e9
+    out = dict(bound="hbm", basis="physical HBM bytes per launch / kernel time",
+               achieved=None if phys is None else round(phys, 1), peak=HBM_PEAK_GBS,
+               unit="GB/s", frac=None if phys is None else round(phys / HBM_PEAK_GBS, 4),
+               traffic=None if tr is None else tr["bytes_per_launch"],
+               kernel=kernel, kernel_ms=round(f["kernel_ms"], 3), walk_ms=round(f["walk_ms"], 3))
+    out["algorithmic"] = dict(bytes_per_step=BYTES_PER_STEP, tet_steps_per_launch=f["tet_steps"],
+                              gbs=round(algo, 1), over_hbm_peak=round(algo / HBM_PEAK_GBS, 3),
+                              reuse=None if phys is None else round(algo / phys, 1))
+    if tr is not None:
+        out["traffic_profile"] = dict(instantiation=tr.get("kernel"), source=tr.get("source"),
+                                      source_stamp=tr.get("source_stamp"), stale=tr["stale"],
+                                      kernel_ms_rocprof=tr.get("kernel_ms_rocprof"))
+        if tr.get("limiter"):
+            out["limiter"] = tr["limiter"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -456,6 +567,8 @@ def main():
     ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=256,
+                    help="default run: requests of the serving leg (0: none)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tess-cache", default=None, help="npz file caching the host tessellation")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -540,26 +653,11 @@ def main():
     value = rays / (f["ms_per_step"] / 1e3)
     kernel = "map_mixed_kernel" if head == "mixed" else "map_fp64_kernel"
     workload = f"{a.config}-{pos.shape[0]} {G}x{G} map, {S} samples/pixel"
-    achieved = f["tet_steps"] * BYTES_PER_STEP / (f["kernel_ms"] / 1e3) / 1e9
-    tr = measured_traffic(kernel, workload)
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=None if tr is None else tr["bytes_per_launch"],
-                    kernel=kernel, bytes_per_step=BYTES_PER_STEP,
-                    tet_steps_per_launch=f["tet_steps"], kernel_ms=round(f["kernel_ms"], 3),
-                    walk_ms=round(f["walk_ms"], 3))
-    if tr is not None:
-        phys = tr["bytes_per_launch"] / (f["kernel_ms"] / 1e3) / 1e9
-        roofline["physical_gbs"] = round(phys, 1)
-        roofline["reuse"] = round(achieved / phys, 2)
-    if achieved > HBM_PEAK_GBS:
-        # cache reuse between neighbouring rays lifts the algorithmic rate
-        # past the HBM peak: the kernel is not HBM-bound
-        roofline["hbm_bound"] = False
-        roofline["note"] = ("algorithmic bytes/s exceed the HBM peak because the rays of a wave "
-                            "re-read the same face records from the vector L1 and L2; physical "
-                            "HBM traffic is physical_gbs; the walk is bound by the vector "
-                            "memory pipeline's data return (two record loads per tet-step)")
+    roofline = roofline_entry(kernel, workload, f)
+
+    serving = None
+    if a.serve_requests > 0:
+        serving = serving_leg(a.serve_requests, a.seed)
 
     cpu = None
     if not a.no_cpu:
@@ -587,7 +685,8 @@ def main():
                    "precision": head},
         "roofline": roofline,
         "cpu_baseline": None if cpu is None else {
-            k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+            **{k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+            "cpu_model": cpu_model()},
         "forms": {k: {"sample_rays_per_s": round(rays / (v["ms_per_step"] / 1e3), 1),
                       "ms_per_step": round(v["ms_per_step"], 3),
                       "walk_ms": round(v["walk_ms"], 3),
@@ -600,6 +699,7 @@ def main():
         "pcie_inclusive": {k: {"ms": round(1e3 * v, 2),
                                "sample_rays_per_s": round(rays / v, 1)}
                            for k, v in dropin.items()},
+        "serving": serving,
     }
     if cpu is not None:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)

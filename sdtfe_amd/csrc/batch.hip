@@ -392,8 +392,9 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
   int rc = ensure_cap(b, n);
   if (rc) return rc;
   const auto t0 = std::chrono::steady_clock::now();
-  // tessellations' setup must be complete before the batch stream reads them
-  for (int i = 0; i < n; ++i) SDTFE_HIP(hipStreamSynchronize(tess[i]->stream));
+  // the tessellations' setup is complete: sdtfe_tess_create returns after
+  // its stream has finished the setup (the stream itself may be a server
+  // pool's, busy with other requests, and is not waited for here)
 
   int tiles = 0;
   for (int i = 0; i < n; ++i) {

@@ -184,7 +184,7 @@ struct DevArray {
 std::once_flag g_pool_kept[64];
 
 int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool given_field,
-           Staging *st, sdtfe_tess **out);
+           Staging *st, hipStream_t stream, sdtfe_tess **out);
 
 }  // namespace
 
@@ -247,8 +247,8 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st)
 }
 
 int tess_create_staged(const double *particle_data, int n, const int *tetra_data, int nt,
-                       float p_mass, Staging *st, sdtfe_tess **out) {
-  return create(particle_data, n, tetra_data, nt, p_mass, false, st, out);
+                       float p_mass, Staging *st, hipStream_t stream, sdtfe_tess **out) {
+  return create(particle_data, n, tetra_data, nt, p_mass, false, st, stream, out);
 }
 
 void link_hull_rows(int *rows, int f) {
@@ -301,12 +301,12 @@ void free_tess(sdtfe_tess *t) {
   if (t->ev1) (void)hipEventDestroy(t->ev1);
   if (t->evk0) (void)hipEventDestroy(t->evk0);
   if (t->evk1) (void)hipEventDestroy(t->evk1);
-  if (t->stream) (void)hipStreamDestroy(t->stream);
+  if (t->stream && t->own_stream) (void)hipStreamDestroy(t->stream);
   delete t;
 }
 
 int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool given_field,
-           Staging *st, sdtfe_tess **out) {
+           Staging *st, hipStream_t stream, sdtfe_tess **out) {
   if (!out) return fail(SDTFE_EINVAL, "null output handle");
   *out = nullptr;
   if (n < 0 || nt < 0) return fail(SDTFE_EINVAL, "negative size");
@@ -320,7 +320,13 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
   t->nt = nt;
   t->p_mass = p_mass;
   t->given_field = given_field;
-  int rc = hip_check(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking), "stream");
+  int rc = 0;
+  if (stream) {
+    t->stream = stream;  // the caller's; it outlives the tessellation
+    t->own_stream = false;
+  } else {
+    rc = hip_check(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking), "stream");
+  }
   if (!rc) rc = hip_check(hipEventCreate(&t->ev0), "event");
   if (!rc) rc = hip_check(hipEventCreate(&t->ev1), "event");
   if (!rc) rc = hip_check(hipEventCreate(&t->evk0), "event");
@@ -377,12 +383,14 @@ uint64_t sdtfe_get_seed(void) { return g_seed.load(); }
 
 int sdtfe_tess_create(const double *particle_data, int n_particles, const int *tetra_data,
                       int n_tetra, float p_mass, sdtfe_tess **out) {
-  return create(particle_data, n_particles, tetra_data, n_tetra, p_mass, false, nullptr, out);
+  return create(particle_data, n_particles, tetra_data, n_tetra, p_mass, false, nullptr, nullptr,
+                out);
 }
 
 int sdtfe_tess_create_field(const double *particle_data, int n_particles, const int *tetra_data,
                             int n_tetra, sdtfe_tess **out) {
-  return create(particle_data, n_particles, tetra_data, n_tetra, 1.0f, true, nullptr, out);
+  return create(particle_data, n_particles, tetra_data, n_tetra, 1.0f, true, nullptr, nullptr,
+                out);
 }
 
 void sdtfe_tess_destroy(sdtfe_tess *t) { free_tess(t); }

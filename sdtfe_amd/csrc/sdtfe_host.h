@@ -42,6 +42,7 @@ struct sdtfe_tess {
   float p_mass = 1.f;
   bool given_field = false;  // vertex values taken from particle_data[:,3]
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false: a stream of the caller's pool (the server's)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evk0 = nullptr, evk1 = nullptr;  // around the walk kernel proper
   // particles and tets as uploaded, plus the field
@@ -110,9 +111,11 @@ struct Staging {
 int staging_init(Staging &st, size_t bytes);
 void staging_free(Staging &st);
 int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st);
-// A tessellation as sdtfe_tess_create makes it, uploading through st.
+// A tessellation as sdtfe_tess_create makes it, uploading through st, on
+// stream (the caller's, kept when the tessellation is destroyed) or, when
+// null, on a stream of its own.
 int tess_create_staged(const double *particle_data, int n, const int *tetra_data, int nt,
-                       float p_mass, Staging *st, sdtfe_tess **out);
+                       float p_mass, Staging *st, hipStream_t stream, sdtfe_tess **out);
 // volume.hip: the volumetric grid's geometry (src/dtfe.c:386-409,
 // 686-695) and its launch (the tessellation's stream)
 VolGeom make_vol_geom(const sdtfe_volume_params &p);

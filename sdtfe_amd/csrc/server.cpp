@@ -6,8 +6,11 @@
 //   setup threads admit queued requests in arrival order while their HBM
 //                 footprint fits the budget and fewer than max_active are
 //                 resident, upload them and build their tessellations
-//                 (sdtfe_tess_create) on their own streams, several requests
-//                 at a time (uploads overlap each other's kernels).
+//                 (sdtfe_tess_create), several requests at a time (uploads
+//                 overlap each other's kernels).  Requests take their
+//                 streams round-robin from a pool of twice the setup threads
+//                 (at most 16), so the streams a server holds stay bounded
+//                 however many requests are resident.
 //   walk thread   repeatedly launches one persistent batch over the next
 //                 tiles of every resident request (at most tile_budget tiles
 //                 per launch, shared round-robin), so a request admitted
@@ -81,6 +84,8 @@ struct sdtfe_server {
   sdtfe::Clock::time_point t0;
   int64_t launches = 0;
   double walk_ms = 0;
+  std::vector<hipStream_t> streams;  // the pool requests set up and copy out on
+  size_t next_stream = 0;            // round-robin cursor (under mu)
 };
 
 namespace sdtfe {
@@ -141,6 +146,7 @@ void setup_loop(sdtfe_server *sv) {
   if (staging_init(st, (size_t)8 << 20)) staging_free(st);  // unpinned uploads then
   for (;;) {
     std::shared_ptr<Request> r;
+    hipStream_t stream = nullptr;
     {
       std::unique_lock<std::mutex> lk(sv->mu);
       sv->cv_setup.wait(lk, [&] {
@@ -162,9 +168,10 @@ void setup_loop(sdtfe_server *sv) {
       sv->used_bytes += r->footprint;
       sv->n_setup++;
       r->t_admit = Clock::now();
+      stream = sv->streams[sv->next_stream++ % sv->streams.size()];
     }
     int rc = tess_create_staged(r->pd.data(), (int)(r->pd.size() / 4), r->tets.data(),
-                                (int)(r->tets.size() / 8), r->p_mass, &st, &r->tess);
+                                (int)(r->tets.size() / 8), r->p_mass, &st, stream, &r->tess);
     // the host copies are uploaded: release them now
     std::vector<double>().swap(r->pd);
     std::vector<int>().swap(r->tets);
@@ -307,6 +314,17 @@ int sdtfe_server_create(const sdtfe_server_config *cfg, sdtfe_server **out) {
   }
   sv->t0 = Clock::now();
   const int n_setup = cfg->setup_threads > 0 ? std::min(cfg->setup_threads, 64) : 4;
+  const int n_streams = std::min(2 * n_setup, 16);
+  for (int i = 0; i < n_streams; ++i) {
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      for (hipStream_t x : sv->streams) (void)hipStreamDestroy(x);
+      sdtfe_batch_destroy(sv->batch);
+      delete sv;
+      return fail(SDTFE_EHIP, "server stream pool");
+    }
+    sv->streams.push_back(st);
+  }
   for (int i = 0; i < n_setup; ++i) sv->setup_thr.emplace_back(setup_loop, sv);
   sv->walk_thr = std::thread(walk_loop, sv);
   *out = sv;
@@ -394,6 +412,10 @@ void sdtfe_server_destroy(sdtfe_server *sv) {
     if (th.joinable()) th.join();
   if (sv->walk_thr.joinable()) sv->walk_thr.join();
   sdtfe_batch_destroy(sv->batch);
+  for (hipStream_t st : sv->streams) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
   delete sv;
 }
 

@@ -37,6 +37,14 @@
 //    The ray's first and last crossings (no neighbour to cancel against; hull
 //    tets are large), crossings of nearly vertical faces and midpoints at the
 //    slab boundary are formed in FP64.
+//  * Error bound.  A crossing's FP32 height and density are off by at most
+//    sum_i k_err M^2 |h_i - h| / den (and the same with densities) to first
+//    order, the weights' sensitivity to their sides' errors.  Since each
+//    crossing is carried into the next tet, its height error cancels except
+//    for the density difference across the two chords; the walk sums these
+//    terms into a bound on the ray's integral and hands the ray to the FP64
+//    walk when the bound exceeds Filter::tol of the value (rays through few,
+//    large, nearly vertical tets near the hull, whose value is small).
 //  * One exit per step, at its top, so the loop's control flow stays simple
 //    (a hand-over decided inside a step is carried to the next one's test).
 //  Per-ray and per-pixel sums stay FP64; volumes and vertex densities are the
@@ -44,6 +52,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "sdtfe_host.h"
 #include "walk_fp64_core.h"
@@ -88,8 +97,13 @@ struct Filter {
   float thr;      // (ZERO + FP64 expression error) / lz
   float k_ill;    // a face whose weight sum is below k_ill M^2 is ill-conditioned
   float k_band;   // crossing height uncertainty over the tet's height range
+  float tol;      // a ray whose error bound exceeds tol of its value goes to FP64
   int full_slab;  // the slab holds the hull's z range: every midpoint is inside
 };
+
+// Rounding allowance per FP32 crossing (heights rebased, the new vertex's
+// height, the mix and its weight sum), in units of the values rounded.
+constexpr float kCrossRound = 4.0f / (1 << 24);
 
 // The filter's bound on a side's FP32 error, plus the zero band.
 SDTFE_HD inline float side_bound(const Filter &flt, float M) {
@@ -109,6 +123,14 @@ SDTFE_HD inline float rcp32(float x) {
 SDTFE_HD inline float side32(float x1, float y1, float x2, float y2) {
   return fmaf(x1, y2, -(y1 * x2));
 }
+
+// |x1 y2| + |y1 x2|: a side's FP32 error is within kSideErr times this
+// (offsets within 2 u of themselves, the product and the fma rounded once
+// each: 4 u |x1 y2| + 5 u |y1 x2| + u |side|, first order).
+SDTFE_HD inline float side_ext(float x1, float y1, float x2, float y2) {
+  return fmaf(fabsf(x1), fabsf(y2), fabsf(y1 * x2));
+}
+constexpr float kSideErr = 6.0f / (1 << 24);
 
 // Largest of two values; NaN-propagating, so the compiler adds no operand
 // canonicalisation and forms three-input maxima (v_maximum3_f32).
@@ -362,6 +384,14 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
   double acc = 0.0;  // sum of (rho_in + rho_out) dz: twice the integral
   int steps = 1;     // tets entered; ~(tets before t) once handed over
   bool slab_fail = false;
+  // Bound on acc's error against the FP64 walk's.  Crossing k's height
+  // error eps_k and density error erho_k enter acc as
+  //   eps_k |a_k - a_{k+1}| + erho_k (d_k + d_{k+1}),
+  // a_k = I_k (rho_{k-1} + rho_k), d_k = I_k |dz_k| (I_k: chord k counts);
+  // the parts with chord k + 1 are added once it is known.  The densities'
+  // own FP32 rounding (kCrossRound rho_k per crossing) sums to exactly
+  // kCrossRound acc and is added at the end.  Crossing 0 is the FP64 one.
+  float E = 0.f, eps_p = 0.f, erho_p = 0.f, a_p = 0.f;
   FaceRec R = load_facerec<kWide>(frec, t, f8);
   for (;;) {
     // exit crossing in FP32, height above the entry crossing (does not wait
@@ -372,21 +402,43 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     float dz = fmaf(e1, p0.z, fmaf(e2, p1.z, e0 * p2.z)) * inv;
     float r_out = fmaf(e1, p0.rho, fmaf(e2, p1.rho, e0 * p2.rho)) * inv;
     const bool stop = (int)R.code < 0 || (unsigned)steps >= (unsigned)g.max_steps;
+    // first-order error of this crossing from its weights: side e (error
+    // within kSideErr side_ext) moves it by (value at the side's vertex -
+    // crossing value) e / den; heights are rounded relative to hab (the
+    // crossing lies within the face's heights, so |dz| <= hab)
+    const float hab = amax(amax(fabsf(p0.z), fabsf(p1.z)), fabsf(p2.z));
+    float q = kSideErr * inv;
     if (!stop && fabsf(den) < flt.k_ill * M * M) {
       // rare: FP32 weights of a nearly vertical face are too uncertain
       const Cross64 o = face_cross64(rec64, t, f8 >> 3, x, y, load_rho4(rec32, t));
       dz = (float)(o.z - (kFullSlab ? carried_z(o, p0, p1, p2) : zc));
       r_out = o.rho;
+      q = 0.f;
     }
     if (stop) break;
     bool ok = true;
+    bool inside = true;
     if (kFullSlab) {
       acc += (double)((rc + r_out) * dz);
     } else {
-      bool inside;
       ok = slab_inside(rec64, rec32, t, enter, f8 >> 3, x, y, zext, zc, zc + (double)dz, g, flt,
                        inside);
       if (inside) acc += (double)((rc + r_out) * dz);
+    }
+    {
+      const float a = self(inside, rc + r_out, 0.f), d = self(inside, fabsf(dz), 0.f);
+      // the entry crossing's terms with this chord
+      E = fmaf(eps_p, fabsf(a_p - a), fmaf(erho_p, d, E));
+      // the exit crossing's, its chord-k part now
+      const float x0 = side_ext(p0.ox, p0.oy, p1.ox, p1.oy);
+      const float x1 = side_ext(p1.ox, p1.oy, p2.ox, p2.oy);
+      const float x2 = side_ext(p2.ox, p2.oy, p0.ox, p0.oy);
+      eps_p = fmaf(q, fmaf(x1, fabsf(p0.z - dz), fmaf(x2, fabsf(p1.z - dz), x0 * fabsf(p2.z - dz))),
+                   2.f * kCrossRound * hab);
+      erho_p = q * fmaf(x1, fabsf(p0.rho - r_out),
+                        fmaf(x2, fabsf(p1.rho - r_out), x0 * fabsf(p2.rho - r_out)));
+      E = fmaf(erho_p, d, E);
+      a_p = a;
     }
     if (!kFullSlab) zc += (double)dz;
     rc = r_out;
@@ -476,6 +528,20 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
       }
     }
     if (inside) acc += (double)(rc + out.rho) * dz;
+    {
+      const float a = self(inside, rc + out.rho, 0.f), d = self(inside, fabsf((float)dz), 0.f);
+      const float hab = amax(amax(fabsf(p0.z), fabsf(p1.z)), fabsf(p2.z));
+      E = fmaf(eps_p, fabsf(a_p - a), fmaf(erho_p, d, E));
+      // this crossing is FP64; dz's reference height comes from the FP32
+      // heights above the carried crossing; the densities' rounding
+      E = fmaf(kCrossRound * (hab + d), a, fmaf(kCrossRound, (float)acc, E));
+    }
+    if (E > flt.tol * (float)acc) {
+      // the FP32 crossings leave the value less certain than tol: the FP64
+      // walk takes the whole ray
+      rs = Resume{entry, -1, 0, 0.0};
+      return false;
+    }
     value = 0.5 * acc;
     crossed = steps;
     return true;
@@ -541,6 +607,11 @@ inline MixedConsts mixed_consts(const sdtfe_tess *t, const MapGeom &g) {
   // 4 k_err / k_ill of the face's height range, itself within the tet's;
   // doubled, plus the FP32 rounding of the mix
   m.flt.k_band = (float)(8.1 * m.flt.k_err / m.flt.k_ill + 1e-6);
+  // per-ray error bound allowed (relative): below the 1e-5 per pixel the
+  // mixed form promises against the FP64 walk, which a pixel's mean of ray
+  // values then keeps; SDTFE_MIXED_TOL overrides it (tests)
+  const char *tol = std::getenv("SDTFE_MIXED_TOL");
+  m.flt.tol = tol && *tol ? (float)std::atof(tol) : 9.5e-6f;
   // every chord midpoint lies within the hull's z range: a midpoint on the
   // boundary belongs to a chord of zero length, which adds nothing
   m.flt.full_slab = t->n > 0 && g.off_z <= t->z_min && g.off_z + g.lz >= t->z_max;

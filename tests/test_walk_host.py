@@ -158,3 +158,35 @@ def test_device_mixed_walk_code(golden, host_walk, tmp_path, slab, wide):
     m = ref != 0
     assert np.max(np.abs(v[m] - ref[m]) / np.abs(ref[m])) <= 1e-5
     assert (v[~m] == 0).all()
+
+
+@pytest.mark.parametrize("n,seed", [(1195, 5), (5175, 14), (3598, 50), (1092, 1)])
+def test_mixed_walk_few_large_tets(host_walk, tmp_path, oracle, n, seed):
+    """Halo cutouts of a few thousand particles: rays near the hull cross a
+    few large, nearly vertical tets and have small values, where the FP32
+    crossings' weight errors are largest.  Every ray is within 1e-5 of the
+    FP64 walk (those whose error bound exceeds the tolerance are handed to
+    the FP64 walk), on random rays and on the rays of pixels that once
+    exceeded it."""
+    from sdtfe_amd import workloads
+    from sdtfe_amd.tessellate import to_particle_data, triangulate
+    pos = workloads.cutout(n, seed)
+    tetra = triangulate(pos)
+    pd = to_particle_data(pos)
+    oracle.density(pd, tetra, 1.0)
+    nabla = oracle.gradients(tetra, pd)
+    tri = oracle.triangles(tetra, pd)
+    rng = np.random.default_rng(seed)
+    x, y = rng.random(3000), rng.random(3000)
+    ent = np.array([oracle.pt_loc_2d(tri, pd, a, b) for a, b in zip(x, y)])
+    m = ent >= 0
+    g = dict(pos=pos, tetra=tetra, rho=pd[:, 3].copy(), nabla=nabla, x=x[m], y=y[m],
+             pixel=np.arange(m.sum()), sample=np.zeros(m.sum(), int), entry=ent[m], lz=1.0,
+             dxy=1.0 / 512, off_z=0.0, seed=seed)
+    want, _ = run_walk(host_walk, tmp_path, g)
+    got, err = run_walk(host_walk, tmp_path, g, mixed=True)
+    np.testing.assert_array_equal(got[:, 1], want[:, 1])
+    v, r = got[:, 0], want[:, 0]
+    k = r != 0
+    assert np.max(np.abs(v[k] - r[k]) / np.abs(r[k])) <= 1e-5
+    assert "fp64_redo" in err
