@@ -248,6 +248,18 @@ int sdtfe_map_device(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_devic
                      sdtfe_stats *stats);
 int sdtfe_tess_sync(sdtfe_tess *t);
 
+/* No counterpart in the reference.  sdtfe_map_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same maps, bit for bit, with
+ * no hipMalloc, hipFree or allocating library call.  Its counters and the
+ * mixed form's second-pass queue live in scratch memory the library holds
+ * per GPU (90,182,400 bytes, about 86 MiB, kept until the process ends).  Limits: a mixed map
+ * holds at most 2^20 pixels (1024 x 1024), else SDTFE_EINVAL; FP64 maps
+ * any size.  Calls on one GPU share that memory and must not overlap (they
+ * race); the tessellation's device must be current. */
+int sdtfe_map_device_noalloc(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_device,
+                             sdtfe_stats *stats);
+
 /* A volumetric grid of the tessellation's vertex field (the densities, or
  * the field given to sdtfe_tess_create_field), as interpolate_to_3d_grid
  * computes it.  Ray (is, js) of column c = i * grid_dim + j draws its

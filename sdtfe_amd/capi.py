@@ -93,6 +93,7 @@ _SIGS = {
                                    C.POINTER(Stats)]),
     "sdtfe_map": (C.c_int, [_vp, C.POINTER(MapParams), _dp, _vp, C.POINTER(Stats)]),
     "sdtfe_map_device": (C.c_int, [_vp, C.POINTER(MapParams), _vp, C.POINTER(Stats)]),
+    "sdtfe_map_device_noalloc": (C.c_int, [_vp, C.POINTER(MapParams), _vp, C.POINTER(Stats)]),
     "sdtfe_tess_sync": (C.c_int, [_vp]),
     "sdtfe_tetra_from_simplices": (C.c_int, [_dp, C.c_int, _ip, C.c_int, _ip]),
     "sdtfe_volume": (C.c_int, [_vp, C.POINTER(VolumeParams), _dp, C.POINTER(Stats)]),
@@ -330,6 +331,13 @@ class Tessellation:
         st = Stats() if stats else None
         check(lib().sdtfe_map_device(self.h, C.byref(params), rho_dev.ptr,
                                      C.byref(st) if st is not None else None))
+        return st.as_dict() if st is not None else None
+
+    def map_device_noalloc(self, params: MapParams, rho_dev: DeviceBuffer, stats: bool = True):
+        """sdtfe_map_device without allocation (the library's scratch memory)."""
+        st = Stats() if stats else None
+        check(lib().sdtfe_map_device_noalloc(self.h, C.byref(params), rho_dev.ptr,
+                                             C.byref(st) if st is not None else None))
         return st.as_dict() if st is not None else None
 
     @staticmethod

@@ -31,7 +31,8 @@ int hip_check(hipError_t e, const char *what) {
   return fail(SDTFE_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate);
+void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate,
+                     unsigned long long *stats);
 void launch_trace_fp64(sdtfe_tess *t, const MapGeom &g, int n, const double *x, const double *y,
                        const int *entry, const uint64_t *pixel, const uint32_t *sample,
                        double *value, int *crossed);
@@ -354,7 +355,7 @@ int run_map(sdtfe_tess *t, const sdtfe_map_params *p, double *out, int *navg, in
     const int rc = launch_map_mixed(t, g, out, navg, accumulate);
     if (rc) return rc;
   } else {
-    launch_map_fp64(t, g, out, navg, accumulate);
+    launch_map_fp64(t, g, out, navg, accumulate, t->stats);
   }
   SDTFE_HIP(hipGetLastError());
   SDTFE_HIP(hipEventRecord(t->ev1, t->stream));
@@ -566,6 +567,36 @@ int sdtfe_map_device(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_devic
     if ((rc = read_stats(t, stats))) return rc;
     stats->t_walk_ms = t_walk;
     stats->t_kernel_ms = t_kernel;
+    stats->t_total_ms = ms_since(t0);
+  }
+  return 0;
+}
+
+int sdtfe_map_device_noalloc(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_device,
+                             sdtfe_stats *stats) {
+  if (!t) return fail(SDTFE_EINVAL, "null tessellation");
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!rho_device) return fail(SDTFE_EINVAL, "null rho");
+  const size_t npix = (size_t)p->grid_dim * (size_t)p->grid_dim;
+  if (p->precision == SDTFE_PREC_MIXED && npix > kNoAllocPixels)
+    return fail(SDTFE_EINVAL, "sdtfe_map_device_noalloc: a mixed map holds at most " +
+                                  std::to_string(kNoAllocPixels) + " pixels");
+  auto t0 = Clock::now();
+  MixedWork w;
+  if ((rc = noalloc_work(&w))) return rc;
+  const MapGeom g = make_geom_pub(t, *p);
+  if ((rc = run_map_noalloc(t, g, p->precision, rho_device, w))) return rc;
+  if (stats) {
+    unsigned long long c[ST_COUNT];
+    SDTFE_HIP(hipMemcpyAsync(c, w.stats, sizeof c, hipMemcpyDeviceToHost, t->stream));
+    SDTFE_HIP(hipStreamSynchronize(t->stream));
+    fill_stats_pub(c, stats);
+    float ms = 0;
+    SDTFE_HIP(hipEventElapsedTime(&ms, t->ev0, t->ev1));
+    stats->t_walk_ms = ms;
+    SDTFE_HIP(hipEventElapsedTime(&ms, t->evk0, t->evk1));
+    stats->t_kernel_ms = ms;
     stats->t_total_ms = ms_since(t0);
   }
   return 0;

@@ -144,4 +144,36 @@ int hull_rows_host(const double *pd, int n, const int *tetra, int nt, const unsi
 void link_hull_rows(int *rows, int n_rows);
 // The status of the calling thread's last drop-in call (0: success).
 void set_dropin_status(int rc);
+
+// walk_mixed.hip: buffers of a mixed map's second pass and the map's
+// counters -- a tessellation's own (grown on demand) or the fixed scratch
+// memory of the entry point that allocates nothing.
+struct MixedWork {
+  unsigned long long *stats;  // ST_COUNT counters
+  Deferred *q;                // queued pixels
+  int *q_count;
+  size_t q_cap;
+  int *off, *tmp;             // per-entry offsets (entries + 1) and scan scratch
+  size_t off_cap, tmp_ints;
+  double *val;                // per handed-over sample: value and whether it counts
+  unsigned char *ok;
+  size_t val_cap;
+};
+// Limits of sdtfe_map_device_noalloc: mixed maps of up to kNoAllocPixels
+// pixels; up to kNoAllocSamples handed-over samples take the per-sample
+// second pass (more take the wave-per-pixel form).  The scratch memory is
+// kNoAllocScratchBytes per GPU.
+constexpr size_t kNoAllocPixels = (size_t)1 << 20;
+constexpr size_t kNoAllocSamples = (size_t)1 << 21;
+constexpr size_t kNoAllocScratchBytes = 256 + 256 + 64 * kNoAllocPixels +
+                                        (((4 * (kNoAllocPixels + 1)) + 255) & ~(size_t)255) +
+                                        4096 + 8 * kNoAllocSamples + kNoAllocSamples;
+// The scratch memory of the current device (its address found once per
+// device) as a MixedWork.
+int noalloc_work(MixedWork *w);
+// A map into device memory (accumulated into) on the tessellation's stream
+// with w's buffers and no allocation: counters cleared by a kernel, then the
+// FP64 map or the mixed map's two passes (grow = false).
+int run_map_noalloc(sdtfe_tess *t, const MapGeom &g, int precision, double *out,
+                    MixedWork &w);
 }  // namespace sdtfe

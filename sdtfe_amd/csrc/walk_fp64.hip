@@ -111,21 +111,23 @@ __global__ void locate_kernel(EntryGrid eg, int n, const double *x, const double
 // ------------------------------------------------------------ launchers
 
 template <int L>
-void launch_map_fp64_l(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
+void launch_map_fp64_l(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate,
+                       unsigned long long *stats) {
   constexpr int tw = 2 * wave_bw(L), th = 2 * wave_bh(L);
   const int nb = ((g.grid + tw - 1) / tw) * ((g.grid + th - 1) / th);
   map_fp64_kernel<L><<<nb, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg, accumulate,
-                                                t->stats);
+                                                stats);
 }
 
-void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
+void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate,
+                     unsigned long long *stats) {
   (void)hipEventRecord(t->evk0, t->stream);
   // at most 4 lanes per pixel: the FP64 walk's registers allow 3 waves per
   // SIMD, and 16 lanes per pixel would need more
   if (map_lanes(g.samples) >= 4)
-    launch_map_fp64_l<4>(t, g, out, navg, accumulate);
+    launch_map_fp64_l<4>(t, g, out, navg, accumulate, stats);
   else
-    launch_map_fp64_l<1>(t, g, out, navg, accumulate);
+    launch_map_fp64_l<1>(t, g, out, navg, accumulate, stats);
   (void)hipEventRecord(t->evk1, t->stream);
 }
 

@@ -2,7 +2,10 @@
 // (walk_mixed_core.h).  A map is two passes: the mixed walk over every
 // pixel, then the FP64 walk over the queued pixels' handed-over samples
 // (walk_defer.h).
+#include <climits>
 #include <cstdlib>
+#include <mutex>
+#include <string>
 
 #include "walk_defer.h"
 
@@ -234,6 +237,8 @@ extern const bool kHaveMixed = true;
 
 namespace {
 
+__global__ void set_int_kernel(int *p, int v) { *p = v; }
+
 int ensure_defer(sdtfe_tess *t, size_t entries) {
   if (!t->defer_count &&
       dev_malloc((void **)&t->defer_count, sizeof(int), t->stream) != hipSuccess)
@@ -248,6 +253,23 @@ int ensure_defer(sdtfe_tess *t, size_t entries) {
   return 0;
 }
 
+// The tessellation's own second-pass buffers, as they stand.
+MixedWork tess_work(sdtfe_tess *t) {
+  MixedWork w{};
+  w.stats = t->stats;
+  w.q = t->defer_buf;
+  w.q_count = t->defer_count;
+  w.q_cap = t->defer_cap;
+  w.off = t->dq_off;
+  w.tmp = t->dq_tmp;
+  w.off_cap = t->dq_off_cap;
+  w.tmp_ints = t->dq_off_cap ? scan_tmp_ints((int64_t)t->dq_off_cap) : 0;
+  w.val = t->dq_val;
+  w.ok = t->dq_ok;
+  w.val_cap = t->dq_val_cap;
+  return w;
+}
+
 // Queues up to this long get a wave per pixel (1024 blocks of 4 waves hold
 // them all at once); longer ones a lane per sample.  SDTFE_DEFER_WAVE_ENTRIES
 // overrides the limit (both forms give the same map; tests run each).
@@ -258,10 +280,28 @@ int defer_wave_entries() {
   return e && *e ? atoi(e) : kDeferWaveEntries;
 }
 
+void launch_deferred_waves(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, double *out,
+                           int *navg, int accumulate, const DeferQueue &dq, int max_entries,
+                           unsigned long long *stats, bool wide) {
+  if (wide)
+    map_deferred_kernel<true><<<1024, 256, 0, t->stream>>>(t->rec32, t->rec64, t->facerec,
+                                                           t->grid, g, mc, out, navg, accumulate,
+                                                           dq, max_entries, stats);
+  else
+    map_deferred_kernel<false><<<1024, 256, 0, t->stream>>>(t->rec32, t->rec64, t->facerec,
+                                                            t->grid, g, mc, out, navg,
+                                                            accumulate, dq, max_entries, stats);
+}
+
+// The second pass with a lane per queued sample over the n queued pixels.
+// grow: the per-entry and per-sample buffers are the tessellation's and grow
+// as needed; otherwise they are fixed, and a queue whose samples exceed them
+// is finished by the wave-per-pixel form (the same map).
 int defer_by_sample(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, double *out,
-                    int *navg, int accumulate, int n, bool wide) {
+                    int *navg, int accumulate, int n, bool wide, MixedWork &w, bool grow) {
   hipStream_t s = t->stream;
-  if ((size_t)n + 1 > t->dq_off_cap) {
+  if ((size_t)n + 1 > w.off_cap || scan_tmp_ints((int64_t)n + 1) > w.tmp_ints) {
+    if (!grow) return fail(SDTFE_EINVAL, "second pass: queue beyond the scratch offsets");
     dev_free(t->dq_off, s);
     dev_free(t->dq_tmp, s);
     t->dq_off = t->dq_tmp = nullptr;
@@ -271,16 +311,26 @@ int defer_by_sample(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, doub
             hipSuccess)
       return fail(SDTFE_ENOMEM, "second-pass offsets allocation failed");
     t->dq_off_cap = (size_t)n + 1;
+    w.off = t->dq_off;
+    w.tmp = t->dq_tmp;
+    w.off_cap = t->dq_off_cap;
+    w.tmp_ints = scan_tmp_ints((int64_t)n + 1);
   }
-  // counts in dq_off[0, n), dq_off[n] = 0, scanned in place into offsets
-  // (dq_off[n] = the number of samples)
-  defer_count_kernel<<<(n + 255) / 256, 256, 0, s>>>(t->defer_buf, n, g.samples, t->dq_off);
-  SDTFE_HIP(hipMemsetAsync(t->dq_off + n, 0, sizeof(int), s));
+  // counts in off[0, n), off[n] = 0, scanned in place into offsets
+  // (off[n] = the number of samples)
+  defer_count_kernel<<<(n + 255) / 256, 256, 0, s>>>(w.q, n, g.samples, w.off);
+  set_int_kernel<<<1, 1, 0, s>>>(w.off + n, 0);
   int total = 0;
-  int rc = scan_exclusive(t->dq_off, t->dq_off, (int64_t)n + 1, t->dq_tmp, &total, s);
+  int rc = scan_exclusive(w.off, w.off, (int64_t)n + 1, w.tmp, &total, s);
   if (rc) return rc;
   if (total <= 0) return fail(SDTFE_EHIP, "second pass: queued pixels list no samples");
-  if ((size_t)total > t->dq_val_cap) {
+  if ((size_t)total > w.val_cap) {
+    if (!grow) {
+      const DeferQueue dq{w.q, w.q_count, (int)w.q_cap};
+      launch_deferred_waves(t, g, mc, out, navg, accumulate, dq, INT_MAX, w.stats, wide);
+      SDTFE_HIP(hipGetLastError());
+      return 0;
+    }
     dev_free(t->dq_val, s);
     dev_free(t->dq_ok, s);
     t->dq_val = nullptr;
@@ -290,97 +340,198 @@ int defer_by_sample(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, doub
         dev_malloc((void **)&t->dq_ok, (size_t)total, s) != hipSuccess)
       return fail(SDTFE_ENOMEM, "second-pass values allocation failed");
     t->dq_val_cap = (size_t)total;
+    w.val = t->dq_val;
+    w.ok = t->dq_ok;
+    w.val_cap = t->dq_val_cap;
   }
   const unsigned nb = (unsigned)((total + 255) / 256);
   if (wide)
     defer_walk_kernel<true><<<nb, 256, 0, s>>>(t->rec32, t->rec64, t->facerec, t->grid, g, mc,
-                                               t->defer_buf, n, t->dq_off, total, t->dq_val,
-                                               t->dq_ok, t->stats);
+                                               w.q, n, w.off, total, w.val, w.ok, w.stats);
   else
     defer_walk_kernel<false><<<nb, 256, 0, s>>>(t->rec32, t->rec64, t->facerec, t->grid, g, mc,
-                                                t->defer_buf, n, t->dq_off, total, t->dq_val,
-                                                t->dq_ok, t->stats);
-  defer_combine_kernel<<<(n + 255) / 256, 256, 0, s>>>(t->defer_buf, n, t->dq_off, t->dq_val,
-                                                       t->dq_ok, out, navg, accumulate);
+                                                w.q, n, w.off, total, w.val, w.ok, w.stats);
+  defer_combine_kernel<<<(n + 255) / 256, 256, 0, s>>>(w.q, n, w.off, w.val, w.ok, out, navg,
+                                                       accumulate);
   SDTFE_HIP(hipGetLastError());
   return 0;
 }
 
 }  // namespace
 
+__global__ void zero_u64_kernel(unsigned long long *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 template <int L>
 void launch_first_pass_l(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, double *out,
-                         int *navg, int accumulate, const DeferQueue &dq) {
+                         int *navg, int accumulate, const DeferQueue &dq,
+                         unsigned long long *stats) {
   constexpr int tw = 2 * wave_bw(L), th = 2 * wave_bh(L);
   const int nb = ((g.grid + tw - 1) / tw) * ((g.grid + th - 1) / th);
   const bool wide = wide_records(t->nt);
   if (mc.flt.full_slab && !wide)
     map_mixed_kernel<true, false, L><<<nb, 256, 0, t->stream>>>(
-        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, stats);
   else if (!wide)
     map_mixed_kernel<false, false, L><<<nb, 256, 0, t->stream>>>(
-        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, stats);
   else if (mc.flt.full_slab)
     map_mixed_kernel<true, true, L><<<nb, 256, 0, t->stream>>>(
-        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, stats);
   else
     map_mixed_kernel<false, true, L><<<nb, 256, 0, t->stream>>>(
-        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, t->stats);
+        t->rec32, t->rec64, t->facerec, t->grid, g, mc, out, navg, accumulate, dq, stats);
 }
 
 void launch_first_pass(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, double *out,
-                       int *navg, int accumulate, const DeferQueue &dq) {
+                       int *navg, int accumulate, const DeferQueue &dq,
+                       unsigned long long *stats) {
   switch (map_lanes(g.samples)) {
-    case 16: launch_first_pass_l<16>(t, g, mc, out, navg, accumulate, dq); break;
-    case 4: launch_first_pass_l<4>(t, g, mc, out, navg, accumulate, dq); break;
-    default: launch_first_pass_l<1>(t, g, mc, out, navg, accumulate, dq); break;
+    case 16: launch_first_pass_l<16>(t, g, mc, out, navg, accumulate, dq, stats); break;
+    case 4: launch_first_pass_l<4>(t, g, mc, out, navg, accumulate, dq, stats); break;
+    default: launch_first_pass_l<1>(t, g, mc, out, navg, accumulate, dq, stats); break;
   }
 }
 
-// The queue holds a quarter of the pixels; a map that overflows it is run
-// again with room for all of them (an accumulating map gets that room from
-// the start, since its first pass cannot be repeated).  The per-pixel second
+// Both passes of a mixed map with the buffers of w.  The per-pixel second
 // pass is queued behind the first and acts only on a short queue; the host
 // reads the queue's length meanwhile and runs the per-sample form for a long
-// one.
+// one.  A queue that overflows w's is grown and the map repeated when grow
+// is set (a map that accumulates must have room for every pixel from the
+// start, since its first pass cannot be repeated), else it is an error.
+int run_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate,
+                  MixedWork &w, bool grow) {
+  const size_t npix = (size_t)g.grid * (size_t)g.grid;
+  const MixedConsts mc = mixed_consts(t, g);
+  const bool wide = wide_records(t->nt);
+  for (;;) {
+    const DeferQueue dq{w.q, w.q_count, (int)std::min(w.q_cap, npix)};
+    defer_reset_kernel<<<1, 1, 0, t->stream>>>(w.q_count);
+    SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
+    launch_first_pass(t, g, mc, out, navg, accumulate, dq, w.stats);
+    SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
+    const int max_wave = defer_wave_entries();
+    launch_deferred_waves(t, g, mc, out, navg, accumulate, dq, max_wave, w.stats, wide);
+    SDTFE_HIP(hipGetLastError());
+    int count = 0;
+    SDTFE_HIP(hipMemcpyAsync(&count, w.q_count, sizeof(int), hipMemcpyDeviceToHost,
+                             t->stream));
+    SDTFE_HIP(hipStreamSynchronize(t->stream));
+    if (count > dq.cap) {
+      // overflow: the queued pixels beyond the queue were not written; the
+      // repeat writes every pixel again
+      if (!grow || accumulate)
+        return fail(SDTFE_EHIP, "second pass: deferred-pixel queue overflowed");
+      zero_u64_kernel<<<1, 64, 0, t->stream>>>(w.stats, ST_COUNT);
+      int rc = ensure_defer(t, npix);
+      if (rc) return rc;
+      w = tess_work(t);
+      continue;
+    }
+    if (count > max_wave)
+      return defer_by_sample(t, g, mc, out, navg, accumulate, count, wide, w, grow);
+    return 0;  // done by the per-pixel pass above (or nothing queued)
+  }
+}
+
+// The queue holds a quarter of the pixels (all of them for a map that
+// accumulates); a map that overflows it is run again with room for all.
 int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
   const size_t npix = (size_t)g.grid * (size_t)g.grid;
   if (npix > (size_t)INT_MAX) return fail(SDTFE_EINVAL, "map too large for the mixed walk");
   const size_t want = accumulate ? npix : std::max(npix / 4, std::min<size_t>(npix, 65536));
   int rc = ensure_defer(t, want);
   if (rc) return rc;
-  const MixedConsts mc = mixed_consts(t, g);
-  const bool wide = wide_records(t->nt);
-  for (;;) {
-    const DeferQueue dq{t->defer_buf, t->defer_count, (int)std::min(t->defer_cap, npix)};
-    defer_reset_kernel<<<1, 1, 0, t->stream>>>(t->defer_count);
-    SDTFE_HIP(hipEventRecord(t->evk0, t->stream));
-    launch_first_pass(t, g, mc, out, navg, accumulate, dq);
-    SDTFE_HIP(hipEventRecord(t->evk1, t->stream));
-    const int max_wave = defer_wave_entries();
-    if (wide)
-      map_deferred_kernel<true><<<1024, 256, 0, t->stream>>>(t->rec32, t->rec64, t->facerec,
-                                                             t->grid, g, mc, out, navg,
-                                                             accumulate, dq, max_wave, t->stats);
-    else
-      map_deferred_kernel<false><<<1024, 256, 0, t->stream>>>(t->rec32, t->rec64, t->facerec,
-                                                              t->grid, g, mc, out, navg,
-                                                              accumulate, dq, max_wave, t->stats);
-    SDTFE_HIP(hipGetLastError());
-    int count = 0;
-    SDTFE_HIP(hipMemcpyAsync(&count, t->defer_count, sizeof(int), hipMemcpyDeviceToHost,
-                             t->stream));
-    SDTFE_HIP(hipStreamSynchronize(t->stream));
-    if (count > dq.cap) {
-      // overflow: the queued pixels beyond the queue were not written; the
-      // repeat writes every pixel again
-      SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
-      if ((rc = ensure_defer(t, npix))) return rc;
-      continue;
+  MixedWork w = tess_work(t);
+  return run_map_mixed(t, g, out, navg, accumulate, w, true);
+}
+
+// ----------------------------------------------- maps without allocation
+// sdtfe_map_device_noalloc keeps its counters, its queue of pixels for the
+// second pass and the second pass's per-entry and per-sample arrays in one
+// file-scope array per GPU, sized for maps of up to kNoAllocPixels pixels
+// (the queue then holds every pixel, so it cannot overflow) and for up to
+// kNoAllocSamples handed-over samples in the per-sample second pass (more
+// are finished by the wave-per-pixel form, which needs no further memory).
+namespace {
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNaStats = 0;
+constexpr size_t kNaCount = kNaStats + align256(sizeof(unsigned long long) * ST_COUNT);
+constexpr size_t kNaQueue = kNaCount + 256;
+constexpr size_t kNaOff = kNaQueue + align256(sizeof(Deferred) * kNoAllocPixels);
+constexpr size_t kNaTmpInts = 1024;  // >= scan_tmp_ints(kNoAllocPixels + 1)
+constexpr size_t kNaTmp = kNaOff + align256(sizeof(int) * (kNoAllocPixels + 1));
+constexpr size_t kNaVal = kNaTmp + align256(sizeof(int) * kNaTmpInts);
+constexpr size_t kNaOk = kNaVal + align256(sizeof(double) * kNoAllocSamples);
+constexpr size_t kNaBytes = kNaOk + align256(kNoAllocSamples);
+static_assert(kNaBytes == kNoAllocScratchBytes, "scratch layout");
+
+}  // namespace
+
+// The scratch memory (device side).  Host code never passes this name to a
+// kernel: each device's address comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+int noalloc_work(MixedWork *w) {
+  static std::mutex mu;
+  static void *addr[64] = {};
+  int dev = 0;
+  SDTFE_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(SDTFE_EINVAL, "device ordinal beyond 63");
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+      // once per device, with that device current
+      void *p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return fail(SDTFE_EHIP, std::string("scratch address: ") + hipGetErrorString(e));
+      addr[dev] = p;
     }
-    if (count > max_wave) return defer_by_sample(t, g, mc, out, navg, accumulate, count, wide);
-    return 0;  // done by the per-pixel pass above (or nothing queued)
+    base = addr[dev];
   }
+  if (scan_tmp_ints((int64_t)kNoAllocPixels + 1) > kNaTmpInts)
+    return fail(SDTFE_EINVAL, "scratch scan space too small");
+  unsigned char *b = static_cast<unsigned char *>(base);
+  *w = MixedWork{};
+  w->stats = reinterpret_cast<unsigned long long *>(b + kNaStats);
+  w->q_count = reinterpret_cast<int *>(b + kNaCount);
+  w->q = reinterpret_cast<Deferred *>(b + kNaQueue);
+  w->q_cap = kNoAllocPixels;
+  w->off = reinterpret_cast<int *>(b + kNaOff);
+  w->off_cap = kNoAllocPixels + 1;
+  w->tmp = reinterpret_cast<int *>(b + kNaTmp);
+  w->tmp_ints = kNaTmpInts;
+  w->val = reinterpret_cast<double *>(b + kNaVal);
+  w->ok = b + kNaOk;
+  w->val_cap = kNoAllocSamples;
+  // SDTFE_NOALLOC_SAMPLES lowers the per-sample limit (tests use it to reach
+  // the wave-per-pixel finish; both give the same map)
+  const char *e = getenv("SDTFE_NOALLOC_SAMPLES");
+  if (e && *e) w->val_cap = std::min<size_t>(w->val_cap, (size_t)std::max(0, atoi(e)));
+  return 0;
+}
+
+void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate,
+                     unsigned long long *stats);
+
+int run_map_noalloc(sdtfe_tess *t, const MapGeom &g, int precision, double *out,
+                    MixedWork &w) {
+  zero_u64_kernel<<<1, 64, 0, t->stream>>>(w.stats, ST_COUNT);
+  SDTFE_HIP(hipEventRecord(t->ev0, t->stream));
+  if (precision == SDTFE_PREC_MIXED) {
+    const int rc = run_map_mixed(t, g, out, nullptr, 1, w, false);
+    if (rc) return rc;
+  } else {
+    launch_map_fp64(t, g, out, nullptr, 1, w.stats);
+  }
+  SDTFE_HIP(hipGetLastError());
+  SDTFE_HIP(hipEventRecord(t->ev1, t->stream));
+  return 0;
 }
 
 void launch_trace_mixed(sdtfe_tess *t, const MapGeom &g, int n, const double *x,
