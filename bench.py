@@ -108,6 +108,8 @@ KERNEL_SOURCES = {
                          "walk_fp64_core.h", "sdtfe_internal.h", "field.hip"],
     "map_fp64_kernel": ["walk_fp64.hip", "walk_fp64_core.h", "walk_common.h",
                         "sdtfe_internal.h", "field.hip"],
+    "volume_rays_kernel+volume_fold_kernel": ["volume.hip", "walk_fp64_core.h", "walk_common.h",
+                                              "sdtfe_internal.h", "field.hip"],
 }
 
 
