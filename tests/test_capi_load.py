@@ -74,3 +74,47 @@ def test_dropin_failure_raises_without_device():
     tet[0, 4:] = -1
     with pytest.raises(capi.SdtfeError, match="no HIP device"):
         capi.compute_density(pd, tet, 8, 1.0, 1.0, np.zeros(64), 1.0, (0.5, 0.5, 0.5), 1, 0.0)
+
+
+def test_link_recipe_keeps_gpu_field_functions(tmp_path):
+    """The link recipe for reference callers: an object that, like the
+    reference's triangulate.o, defines triangulate beside pre_compute_vol and
+    gradients, with those two made local by objcopy, leaves them undefined in
+    the program, so they resolve to libsdtfe_amd.so; without the objcopy step
+    the program's own copies would be used."""
+    import subprocess
+    libdir = os.path.dirname(capi.LIB_PATH)
+    (tmp_path / "tri.c").write_text(
+        "#include <stddef.h>\n"
+        "int *triangulate(double *p, int n, size_t *nf, char *o) { *nf = 0; return 0; }\n"
+        "void pre_compute_vol(double **p, int n, int *t, int nt, float m) {}\n"
+        "double *gradients(int *t, int nt, double *p) { return 0; }\n")
+    (tmp_path / "main.c").write_text(
+        "#include <stddef.h>\n"
+        "int *triangulate(double *, int, size_t *, char *);\n"
+        "void pre_compute_vol(double **, int, int *, int, float);\n"
+        "double *gradients(int *, int, double *);\n"
+        "int main(void) { size_t n; triangulate(0, 0, &n, 0); pre_compute_vol(0, 0, 0, 0, 1.f);"
+        " gradients(0, 0, 0); return 0; }\n")
+
+    def run(*cmd):
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=tmp_path)
+        assert r.returncode == 0, r.stderr
+        return r.stdout
+
+    run("gcc", "-c", "tri.c", "-o", "tri.o")
+    run("gcc", "-c", "main.c", "-o", "main.o")
+    run("objcopy", "--localize-symbol=pre_compute_vol", "--localize-symbol=gradients",
+        "tri.o", "tri_q.o")
+    for obj, local in (("tri_q.o", True), ("tri.o", False)):
+        run("gcc", "main.o", obj, "-L" + libdir, "-lsdtfe_amd", "-Wl,-rpath," + libdir,
+            "-o", "prog")
+        syms = run("nm", "prog")
+        defined = {ln.split()[-1] for ln in syms.splitlines() if " T " in ln}
+        undefined = {ln.split()[-1] for ln in syms.splitlines() if " U " in ln}
+        assert "triangulate" in defined
+        for name in ("pre_compute_vol", "gradients"):
+            if local:
+                assert name in undefined and name not in defined, syms
+            else:
+                assert name in defined, syms
