@@ -356,6 +356,8 @@ int capture(sdtfe_batch *b, int blocks, size_t lds) {
 
 namespace sdtfe {
 
+int batch_reserve(sdtfe_batch *b, int n) { return ensure_cap(b, n); }
+
 int batch_tiles(const sdtfe_map_params &p, int lanes_per_pixel) {
   int L = lanes_per_pixel;
   if (L != 1 && L != 2 && L != 4 && L != 8 && L != 16) L = 1;

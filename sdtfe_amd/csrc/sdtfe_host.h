@@ -94,6 +94,9 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
                      const sdtfe_map_params *params, double *const *rho_device,
                      const int *lanes_per_pixel, const int *tile_begin, const int *tile_count,
                      sdtfe_stats *item_stats, sdtfe_stats *total);
+// Sizes a batch's item buffers for n items up front, so later launches of up
+// to n items neither reallocate them nor capture the graph again.
+int batch_reserve(sdtfe_batch *b, int n);
 // Stream-ordered device memory from the device's default pool, which keeps
 // freed blocks for reuse (capi.cpp): unlike hipMalloc / hipFree these do not
 // synchronise the device, so tessellations set up or freed on one stream do

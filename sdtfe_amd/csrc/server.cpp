@@ -307,8 +307,13 @@ int sdtfe_server_create(const sdtfe_server_config *cfg, sdtfe_server **out) {
     return fail(SDTFE_EINVAL, "unknown precision");
   sdtfe_server *sv = new sdtfe_server();
   sv->cfg = *cfg;
+  // item buffers for max_active requests now: the walk thread then never
+  // reallocates them or captures the batch graph again while set-up threads
+  // are issuing work
   int rc = sdtfe_batch_create(&sv->batch);
+  if (!rc) rc = batch_reserve(sv->batch, cfg->max_active);
   if (rc) {
+    if (sv->batch) sdtfe_batch_destroy(sv->batch);
     delete sv;
     return rc;
   }

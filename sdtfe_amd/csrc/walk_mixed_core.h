@@ -38,8 +38,9 @@
 //    tets are large), crossings of nearly vertical faces and midpoints at the
 //    slab boundary are formed in FP64.
 //  * Error bound.  A crossing's FP32 height and density are off by at most
-//    sum_i k_err M^2 |h_i - h| / den (and the same with densities) to first
-//    order, the weights' sensitivity to their sides' errors.  Since each
+//    sum_i kSideErr side_ext_i |h_i - h| / den (and the same with densities)
+//    to first order, the weights' sensitivity to their sides' errors, where
+//    side_ext_i = |x1 y2| + |y1 x2| of side i's own offsets.  Since each
 //    crossing is carried into the next tet, its height error cancels except
 //    for the density difference across the two chords; the walk sums these
 //    terms into a bound on the ray's integral and hands the ray to the FP64
