@@ -106,7 +106,7 @@ def time_form(tess, cfg, precision, steps, warmup, seed):
 KERNEL_SOURCES = {
     "map_mixed_kernel": ["walk_mixed.hip", "walk_mixed_core.h", "walk_defer.h", "walk_common.h",
                          "walk_fp64_core.h", "sdtfe_internal.h", "field.hip"],
-    "map_fp64_kernel": ["walk_fp64.hip", "walk_fp64_core.h", "walk_common.h",
+    "map_fp64_kernel": ["walk_fp64.hip", "walk_fp64_face.h", "walk_fp64_core.h", "walk_common.h",
                         "sdtfe_internal.h", "field.hip"],
     "volume_rays_kernel+volume_fold_kernel": ["volume.hip", "walk_fp64_core.h", "walk_common.h",
                                               "sdtfe_internal.h", "field.hip"],

@@ -24,6 +24,7 @@
 
 #include "sdtfe_host.h"
 #include "walk_defer.h"
+#include "walk_fp64_face.h"
 
 namespace sdtfe {
 
@@ -34,6 +35,7 @@ struct BatchItem {
   const Rec64 *rec64;
   const Rec32 *rec32;
   const FaceRec *facerec;
+  const FRec64 *frec64;
   EntryGrid grid;
   MapGeom geom;
   MixedConsts mc;
@@ -86,7 +88,7 @@ __device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int
   cnt.samples++;
   v = 0.0;
   crossed = 0;
-  const int entry = locate_entry(it.grid, xt, yt);
+  const int entry = kMixed ? locate_entry(it.grid, xt, yt) : locate_entry_rid(it.grid, xt, yt);
   if (entry < 0) {
     cnt.missed++;
     return true;
@@ -100,7 +102,7 @@ __device__ inline bool sample_one(const BatchItem &it, const uint8_t *faces, int
                                                      xt, yt, g, it.mc.flt, v, crossed, cnt, rs);
     if (!done) return false;
   } else {
-    v = walk64(it.rec64, entry, xt, yt, g, key, crossed, cnt);
+    v = walk64f(it.rec64, it.frec64, entry, xt, yt, g, key, crossed, cnt);
   }
   cnt.tet_steps += (unsigned)crossed;
   if (crossed == 0) cnt.failed++;
@@ -380,6 +382,8 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
   for (int i = 0; i < n; ++i) {
     if (!tess[i] || !rho_device[i]) return fail(SDTFE_EINVAL, "null item");
     if (params[i].precision != prec) return fail(SDTFE_EINVAL, "mixed precisions in one batch");
+    if (prec == SDTFE_PREC_FP64 && !tess[i]->frec64)
+      return fail(SDTFE_EINVAL, "batched FP64 maps take tessellations of fewer than 2^29 tets");
     if (params[i].grid_dim <= 0 || params[i].grid_dim > 65536 || params[i].n_mc_samp < 0 ||
         !(params[i].box_len > 0) || !(params[i].box_len_z > 0))
       return fail(SDTFE_EINVAL, "bad map parameters");
@@ -406,6 +410,7 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
     it.rec64 = tess[i]->rec64;
     it.rec32 = tess[i]->rec32;
     it.facerec = tess[i]->facerec;
+    it.frec64 = tess[i]->frec64;
     it.grid = tess[i]->grid;
     it.geom = make_geom_pub(tess[i], p);
     it.mc = mixed_consts(tess[i], it.geom);

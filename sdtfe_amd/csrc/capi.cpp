@@ -185,7 +185,7 @@ struct DevArray {
 std::once_flag g_pool_kept[64];
 
 int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool given_field,
-           Staging *st, hipStream_t stream, sdtfe_tess **out);
+           Staging *st, hipStream_t stream, sdtfe_tess **out, bool fp64_faces = true);
 
 }  // namespace
 
@@ -248,8 +248,9 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st)
 }
 
 int tess_create_staged(const double *particle_data, int n, const int *tetra_data, int nt,
-                       float p_mass, Staging *st, hipStream_t stream, sdtfe_tess **out) {
-  return create(particle_data, n, tetra_data, nt, p_mass, false, st, stream, out);
+                       float p_mass, Staging *st, hipStream_t stream, bool fp64_faces,
+                       sdtfe_tess **out) {
+  return create(particle_data, n, tetra_data, nt, p_mass, false, st, stream, out, fp64_faces);
 }
 
 void link_hull_rows(int *rows, int f) {
@@ -290,7 +291,7 @@ namespace {
 void free_tess(sdtfe_tess *t) {
   if (!t) return;
   void *bufs[] = {t->pos4,    t->tets,     t->vol,     t->nabla,   t->rec64,
-                  t->rec32,   t->facerec,   t->tri_xy,   t->tri_tet, t->tri_abc, t->cell_off,
+                  t->rec32,   t->facerec,   t->frec64,   t->tri_xy,   t->tri_tet, t->tri_rid, t->tri_abc, t->cell_off,
                   t->cell_tri, t->stats,   t->map_buf, t->navg_buf, t->defer_buf,
                   t->defer_count, t->dq_off, t->dq_tmp, t->dq_val, t->dq_ok};
   if (t->stream) {
@@ -307,7 +308,7 @@ void free_tess(sdtfe_tess *t) {
 }
 
 int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool given_field,
-           Staging *st, hipStream_t stream, sdtfe_tess **out) {
+           Staging *st, hipStream_t stream, sdtfe_tess **out, bool fp64_faces) {
   if (!out) return fail(SDTFE_EINVAL, "null output handle");
   *out = nullptr;
   if (n < 0 || nt < 0) return fail(SDTFE_EINVAL, "negative size");
@@ -321,6 +322,7 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
   t->nt = nt;
   t->p_mass = p_mass;
   t->given_field = given_field;
+  t->fp64_faces = fp64_faces;
   int rc = 0;
   if (stream) {
     t->stream = stream;  // the caller's; it outlives the tessellation

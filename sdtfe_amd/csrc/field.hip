@@ -17,6 +17,7 @@
 
 #include "sdtfe_host.h"
 #include "walk_common.h"
+#include "walk_fp64_face.h"
 
 namespace sdtfe {
 
@@ -283,6 +284,16 @@ __global__ void face_record_kernel(const double4 *pos, const int *tets, int nt, 
   out[i] = r;
 }
 
+// FP64 walk records of every (tet, face).
+__global__ void face_record64_kernel(const double4 *pos, const int *tets, const double *nabla,
+                                     int nt, FRec64 *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4 * (int64_t)nt) return;
+  FRec64 r;
+  make_frec64(pos, tets, nabla, (int)(i >> 2), (int)(i & 3), r);
+  out[i] = r;
+}
+
 // ============================================================ entry grid
 
 // Face f of a tet as the triangle the reference emits (src/triangulate.c:
@@ -319,7 +330,7 @@ __global__ void hull_mask_kernel(const double4 *pos, const int *tets, int nt,
 
 __global__ void emit_tri_kernel(const double4 *pos, const int *tets, int nt,
                                 const unsigned char *mask, const int *off, double *tri_xy,
-                                int *tri_tet, int *tri_abc) {
+                                int *tri_tet, int *tri_rid, int *tri_abc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   const unsigned m = mask[t];
@@ -337,6 +348,7 @@ __global__ void emit_tri_kernel(const double4 *pos, const int *tets, int nt,
     tri_abc[3 * k + 1] = i1;
     tri_abc[3 * k + 2] = i2;
     tri_tet[k] = t;
+    tri_rid[k] = nt < (1 << 29) ? 4 * t + f : -1;
     ++k;
   }
 }
@@ -498,6 +510,12 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra, Staging *st) 
     face_record_kernel<<<blocks_for(4 * (int64_t)nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->facerec);
   }
+  // face records of the FP64 walk, addressed as 4 x tet + face in an int
+  if (t->fp64_faces && nt > 0 && nt < (1 << 29)) {
+    SDTFE_RC(dev_alloc(t, &t->frec64, 4 * (size_t)nt));
+    face_record64_kernel<<<blocks_for(4 * (int64_t)nt, 256), 256, 0, s>>>(
+        reinterpret_cast<const double4 *>(t->pos4), t->tets, t->nabla, nt, t->frec64);
+  }
   SDTFE_HIP(hipGetLastError());
   SDTFE_HIP(hipStreamSynchronize(s));
   return 0;
@@ -521,15 +539,17 @@ int build_entry_grid(sdtfe_tess *t) {
   t->n_tri = n_tri;
   SDTFE_RC(dev_alloc(t, &t->tri_xy, 6 * (size_t)n_tri));
   SDTFE_RC(dev_alloc(t, &t->tri_tet, (size_t)n_tri));
+  SDTFE_RC(dev_alloc(t, &t->tri_rid, (size_t)n_tri));
   SDTFE_RC(dev_alloc(t, &t->tri_abc, 3 * (size_t)n_tri));
   if (nt > 0)
     emit_tri_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, mask, off, t->tri_xy,
-        t->tri_tet, t->tri_abc);
+        t->tri_tet, t->tri_rid, t->tri_abc);
 
   EntryGrid g{};
   g.tri_xy = t->tri_xy;
   g.tri_tet = t->tri_tet;
+  g.tri_rid = t->tri_rid;
   g.n_tri = n_tri;
   g.nx = g.ny = 1;
   double bb[4] = {0, 0, 0, 0};
@@ -651,18 +671,20 @@ int hull_rows_host(const double *pd, int n, const int *tetra, int nt, const unsi
   SDTFE_RC(h.init(pd, n, tetra, nt));
   Scratch sc(h.s);
   unsigned char *dmask = nullptr;
-  int *off = nullptr, *dtet = nullptr, *dabc = nullptr;
+  int *off = nullptr, *dtet = nullptr, *drid = nullptr, *dabc = nullptr;
   double *dxy = nullptr;
   SDTFE_RC(sc.get(&dmask, (size_t)nt));
   SDTFE_RC(sc.get(&off, (size_t)nt));
   SDTFE_RC(sc.get(&dtet, (size_t)n_faces));
+  SDTFE_RC(sc.get(&drid, (size_t)n_faces));
   SDTFE_RC(sc.get(&dabc, 3 * (size_t)n_faces));
   SDTFE_RC(sc.get(&dxy, 6 * (size_t)n_faces));
   SDTFE_HIP(hipMemcpyAsync(dmask, mask, (size_t)nt, hipMemcpyHostToDevice, h.s));
   SDTFE_HIP(hipMemcpyAsync(off, off_h.data(), sizeof(int) * (size_t)nt, hipMemcpyHostToDevice,
                            h.s));
   emit_tri_kernel<<<blocks_for(nt, 256), 256, 0, h.s>>>(reinterpret_cast<const double4 *>(h.pos),
-                                                        h.tets, nt, dmask, off, dxy, dtet, dabc);
+                                                        h.tets, nt, dmask, off, dxy, dtet, drid,
+                                                        dabc);
   SDTFE_HIP(hipGetLastError());
   SDTFE_HIP(hipMemcpyAsync(abc, dabc, sizeof(int) * 3 * (size_t)n_faces, hipMemcpyDeviceToHost,
                            h.s));

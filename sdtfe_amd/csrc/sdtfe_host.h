@@ -41,6 +41,7 @@ struct sdtfe_tess {
   int n = 0, nt = 0, n_tri = 0;
   float p_mass = 1.f;
   bool given_field = false;  // vertex values taken from particle_data[:,3]
+  bool fp64_faces = true;    // build the FP64 walk's face records
   hipStream_t stream = nullptr;
   bool own_stream = true;  // false: a stream of the caller's pool (the server's)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -53,12 +54,14 @@ struct sdtfe_tess {
   sdtfe::Rec64 *rec64 = nullptr;
   sdtfe::Rec32 *rec32 = nullptr;
   sdtfe::FaceRec *facerec = nullptr;  // 4 per tet
+  sdtfe::FRec64 *frec64 = nullptr;    // 4 per tet; null beyond 2^29 tets or when not built
   double org[3] = {0, 0, 0};  // centre of the particles' bounding box
   double span = 0;            // its half-extent (max over axes)
   double z_min = 0, z_max = 0;  // the particles' (and the hull's) z range
   // entry structure
   double *tri_xy = nullptr;
   int *tri_tet = nullptr;
+  int *tri_rid = nullptr;     // F: 4 x tet + face, with face records
   int *tri_abc = nullptr;     // F x 3 vertex ids, for inspection
   int *cell_off = nullptr;
   int *cell_tri = nullptr;
@@ -116,9 +119,11 @@ void staging_free(Staging &st);
 int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st);
 // A tessellation as sdtfe_tess_create makes it, uploading through st, on
 // stream (the caller's, kept when the tessellation is destroyed) or, when
-// null, on a stream of its own.
+// null, on a stream of its own; without the FP64 walk's face records when
+// fp64_faces is false (FP64 maps then walk the per-tet records).
 int tess_create_staged(const double *particle_data, int n, const int *tetra_data, int nt,
-                       float p_mass, Staging *st, hipStream_t stream, sdtfe_tess **out);
+                       float p_mass, Staging *st, hipStream_t stream, bool fp64_faces,
+                       sdtfe_tess **out);
 // volume.hip: the volumetric grid's geometry (src/dtfe.c:386-409,
 // 686-695) and its launch (the tessellation's stream)
 VolGeom make_vol_geom(const sdtfe_volume_params &p);

@@ -29,6 +29,37 @@ struct alignas(16) Rec64 {
 };
 static_assert(sizeof(Rec64) == 144, "Rec64 layout");
 
+// FP64 walk record of a tet seen from one of its faces, one per (tet t,
+// face f), 144 B read as nine 16-byte loads: what a ray entering t through
+// face f (the face opposite vertex f) needs to cross t and leave it.
+//   n     vertex f (N): x, y, z
+//   p     face f's three vertices P0, P1, P2 (x, y, z each) in the order the
+//         reference forms that face's barycentric height (src/dtfe.c:67-213):
+//         f0 BDC: D, C, B; f1 ACD: C, D, A; f2 ADB: B, A, D; f3 ABC: A, B, C
+//   rho_a, g  the density at vertex A and the tet's gradient
+//   next  for k = 0..2, the face opposite P_k (one of the faces meeting at
+//         N): 4 x the tet across it + that face's index there, -1 on the
+//         hull, -2 when the mesh lacks the back link
+//   code  bits 0-1: which of face f's sides its denominator adds last;
+//         bits 2+4k (2 bits): rotation of the face opposite P_k (its order
+//         in the reference is (P_{k+2}, P_{k+1}, N) rotated left by this);
+//         bits 4+4k (2 bits): which of that face's sides is added last;
+//         bits 14+2i (2 bits): position of P_i in the reference order of
+//         the same face in the tet across f; bit 30: no such position (the
+//         ray's sides are then formed afresh); bit 31: a ray entering here
+//         is handed to the walk over Rec64 (missing or ambiguous links, or
+//         the two tets do not see the face with opposite orientations)
+struct alignas(16) FRec64 {
+  double n[3];
+  double p[9];
+  double rho_a, g[3];
+  int next[3];
+  unsigned code;
+};
+static_assert(sizeof(FRec64) == 144, "FRec64 layout");
+constexpr unsigned kFRecFresh = 1u << 30;
+constexpr unsigned kFRecHand = 1u << 31;
+
 // Mixed-precision walk record, 96 B: neighbours, vertex A in FP64 and the
 // edges B - A, C - A, D - A plus the four vertex densities in FP32.  Offsets
 // from a ray are built from A in FP64, so FP32 rounding stays a fraction of
@@ -95,6 +126,7 @@ struct DeferQueue {
 struct EntryGrid {
   const double *tri_xy;   // F x 6: a.x a.y b.x b.y c.x c.y
   const int *tri_tet;     // F
+  const int *tri_rid;     // F: 4 x tet + face (the face record a ray enters by)
   const int *cell_off;    // nx*ny + 1
   const int *cell_tri;    // triangle ids per cell
   double x0, y0, x1, y1;  // grid extent (bounding box of the triangles)

@@ -57,11 +57,12 @@ struct Request {
   sdtfe_stats stats{};
 };
 
-int64_t estimate_footprint(int n, int nt, int grid) {
+int64_t estimate_footprint(int n, int nt, int grid, bool fp64) {
   // resident: pos4 32 B/vertex; per tet: row 32, volume 8, gradient 24,
-  // records 144 (FP64) + 96 (mixed) + 4 x 32 (faces); map 8 B/pixel; setup
-  // scratch ~ 16 B/tet + 12 B/vertex
-  return (int64_t)n * 44 + (int64_t)nt * 448 + (int64_t)grid * grid * 8 + (1 << 20);
+  // records 144 (FP64) + 96 (mixed) + 4 x 32 (faces) and, for FP64 maps,
+  // 4 x 144 (FP64 faces); map 8 B/pixel; setup scratch ~ 16 B/tet + 12 B/vertex
+  return (int64_t)n * 44 + (int64_t)nt * (fp64 ? 1024 : 448) + (int64_t)grid * grid * 8 +
+         (1 << 20);
 }
 
 }  // namespace
@@ -170,8 +171,10 @@ void setup_loop(sdtfe_server *sv) {
       r->t_admit = Clock::now();
       stream = sv->streams[sv->next_stream++ % sv->streams.size()];
     }
+    // a mixed-precision server never walks the FP64 face records
     int rc = tess_create_staged(r->pd.data(), (int)(r->pd.size() / 4), r->tets.data(),
-                                (int)(r->tets.size() / 8), r->p_mass, &st, stream, &r->tess);
+                                (int)(r->tets.size() / 8), r->p_mass, &st, stream,
+                                sv->cfg.precision == SDTFE_PREC_FP64, &r->tess);
     // the host copies are uploaded: release them now
     std::vector<double>().swap(r->pd);
     std::vector<int>().swap(r->tets);
@@ -353,7 +356,8 @@ int sdtfe_server_submit(sdtfe_server *sv, const double *particle_data, int n_par
   r->p_mass = p_mass;
   r->params = *p;
   r->out = rho;
-  r->footprint = estimate_footprint(n_particles, n_tetra, p->grid_dim);
+  r->footprint = estimate_footprint(n_particles, n_tetra, p->grid_dim,
+                                    p->precision == SDTFE_PREC_FP64);
   r->t_submit = Clock::now();
   std::lock_guard<std::mutex> g(sv->mu);
   r->id = sv->next_id++;

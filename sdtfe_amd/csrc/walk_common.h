@@ -34,7 +34,7 @@ SDTFE_HD inline bool tri_contains(const double *xy, double qx, double qy) {
 // -1 outside the hull's shadow.  Replaces the visibility walk of pt_loc_2d
 // (src/dtfe.c:518-552); both return the same tet except for points on a
 // shared edge, where either neighbour is a valid entry.
-SDTFE_HD inline int locate_entry(const EntryGrid &g, double x, double y) {
+SDTFE_HD inline int locate_tri(const EntryGrid &g, double x, double y) {
   if (g.n_tri == 0) return -1;
   if (!(x >= g.x0 && x <= g.x1 && y >= g.y0 && y <= g.y1)) return -1;
   const int c = grid_cell(y, g.y0, g.sy, g.ny) * g.nx + grid_cell(x, g.x0, g.sx, g.nx);
@@ -44,7 +44,18 @@ SDTFE_HD inline int locate_entry(const EntryGrid &g, double x, double y) {
     const int k = g.cell_tri[i];
     if (k < best && tri_contains(g.tri_xy + 6 * (int64_t)k, x, y)) best = k;
   }
-  return best == INT_MAX ? -1 : g.tri_tet[best];
+  return best == INT_MAX ? -1 : best;
+}
+
+SDTFE_HD inline int locate_entry(const EntryGrid &g, double x, double y) {
+  const int k = locate_tri(g, x, y);
+  return k < 0 ? -1 : g.tri_tet[k];
+}
+
+// The same entry as the face record the ray enters by (4 x tet + face).
+SDTFE_HD inline int locate_entry_rid(const EntryGrid &g, double x, double y) {
+  const int k = locate_tri(g, x, y);
+  return k < 0 ? -1 : g.tri_rid[k];
 }
 
 // Lower-left corner of pixel (px, py)'s sampling square and one sample in

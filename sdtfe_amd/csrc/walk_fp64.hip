@@ -10,8 +10,11 @@
 // single dependent load.  A map launch spreads each pixel's samples over 4
 // lanes (4 or more samples), so the 64 rays of a wave start together on a
 // 4x4-pixel patch of the map and cross mostly the same tets.
+#include <cstdlib>
+#include <string>
+
 #include "sdtfe_host.h"
-#include "walk_fp64_core.h"
+#include "walk_fp64_face.h"
 
 namespace sdtfe {
 
@@ -20,11 +23,12 @@ namespace sdtfe {
 // sample order after each round, so pixel value = sum / n_avg is the same
 // for every L (src/dtfe.c:613-645).  accumulate: out += value, else out =
 // value.
-template <int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void map_fp64_kernel(const Rec64 *__restrict__ rec,
-                                                      EntryGrid eg, MapGeom g, double *out,
-                                                      int *navg, int accumulate,
-                                                      unsigned long long *stats) {
+// kFace: the walk over face records (walk64f), else over Rec64 (walk64);
+// the same results.
+template <int L, bool kFace>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void map_fp64_kernel(
+    const Rec64 *__restrict__ rec, const FRec64 *__restrict__ frec, EntryGrid eg, MapGeom g,
+    double *out, int *navg, int accumulate, unsigned long long *stats) {
   int px, py, j;
   split_pixel<L>(g.grid, px, py, j);
   const bool active = px < g.grid && py < g.grid;
@@ -40,13 +44,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
     sample_point(g, px, py, key, xt, yt);
     cnt.samples++;
     v = 0.0;
-    const int entry = locate_entry(eg, xt, yt);
+    const int entry = kFace ? locate_entry_rid(eg, xt, yt) : locate_entry(eg, xt, yt);
     if (entry < 0) {
       cnt.missed++;
       return 0;
     }
     int crossed;
-    v = walk64(rec, entry, xt, yt, g, key, crossed, cnt);
+    v = kFace ? walk64f(rec, frec, entry, xt, yt, g, key, crossed, cnt)
+              : walk64(rec, entry, xt, yt, g, key, crossed, cnt);
     cnt.tet_steps += (unsigned)crossed;
     if (crossed == 0) cnt.failed++;
     return crossed > 0 ? 1 : 2;
@@ -110,21 +115,33 @@ __global__ void locate_kernel(EntryGrid eg, int n, const double *x, const double
 
 // ------------------------------------------------------------ launchers
 
+// SDTFE_FP64_WALK=rec takes the walk over Rec64 for every FP64 map (the two
+// walks give the same maps; tests run both).
+bool fp64_face_walk() {
+  const char *e = getenv("SDTFE_FP64_WALK");
+  return !(e && std::string(e) == "rec");
+}
+
 template <int L>
 void launch_map_fp64_l(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate,
                        unsigned long long *stats) {
   constexpr int tw = 2 * wave_bw(L), th = 2 * wave_bh(L);
   const int nb = ((g.grid + tw - 1) / tw) * ((g.grid + th - 1) / th);
-  map_fp64_kernel<L><<<nb, 256, 0, t->stream>>>(t->rec64, t->grid, g, out, navg, accumulate,
-                                                stats);
+  if (t->frec64 && fp64_face_walk())
+    map_fp64_kernel<L, true><<<nb, 256, 0, t->stream>>>(t->rec64, t->frec64, t->grid, g, out,
+                                                        navg, accumulate, stats);
+  else
+    map_fp64_kernel<L, false><<<nb, 256, 0, t->stream>>>(t->rec64, nullptr, t->grid, g, out,
+                                                         navg, accumulate, stats);
 }
 
 void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate,
                      unsigned long long *stats) {
   (void)hipEventRecord(t->evk0, t->stream);
-  // at most 4 lanes per pixel: the FP64 walk's registers allow 3 waves per
-  // SIMD, and 16 lanes per pixel would need more
-  if (map_lanes(g.samples) >= 4)
+  const int lanes = map_lanes(g.samples);
+  if (lanes == 16)
+    launch_map_fp64_l<16>(t, g, out, navg, accumulate, stats);
+  else if (lanes >= 4)
     launch_map_fp64_l<4>(t, g, out, navg, accumulate, stats);
   else
     launch_map_fp64_l<1>(t, g, out, navg, accumulate, stats);

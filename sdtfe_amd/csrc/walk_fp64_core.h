@@ -211,15 +211,17 @@ SDTFE_HD inline double tet_integral64(const Tet64 &r, double ox, double oy, doub
 // the entry tet: start = from = entry, steps0 = 0, which matches the
 // reference's dummy first record).  key names the sample; perturbation
 // corners are draws 2, 3, ... of it.  crossed: the tets crossed by this
-// call (0 when the walk fails).
+// call (0 when the walk fails).  acc0: the sum of the tets crossed before,
+// which the walk adds to in order (0 when it starts at the entry tet).
 SDTFE_HD inline double walk64_from(const Rec64 *__restrict__ rec, int start, int from,
                                    int steps0, double x, double y, const MapGeom &g,
-                                   uint64_t key, int &crossed, LaneCounters &cnt) {
+                                   uint64_t key, int &crossed, LaneCounters &cnt,
+                                   double acc0 = 0.0) {
   crossed = 0;
   if (start < 0) return 0.0;
   const double max_dist = g.dxy / 2.0;
   const double step = max_dist / kPerturbMax;
-  double ox = x, oy = y, acc = 0.0;
+  double ox = x, oy = y, acc = acc0;
   uint32_t draw = 2;
   int cur = from, prev = -1, f_in = -1, f_out = 0, nxt = start, steps = steps0;
   bool first = true;

@@ -2,7 +2,11 @@
 // walk_fp64_core.h): the same functions compiled for the CPU, driven by
 // tests/test_walk_host.py.  Test infrastructure; the library never uses it.
 // Input: a binary file with records and rays; output: (value, crossed).
+// Modes: FP64 walk over Rec64 (default), "face" (the FP64 walk over face
+// records built by make_frec64 from the particles and tetra_data rows that
+// follow the rays), "mixed" / "mixedwide" (the mixed walk).
 #include "walk_fp64.hip"
+#include "walk_fp64_face.h"
 #include "walk_mixed_core.h"
 
 #include <cstdio>
@@ -30,8 +34,10 @@ static int check_faces() {
 int main(int argc, char **argv) {
   if (argc == 2 && std::string(argv[1]) == "faces") return check_faces();
   if (argc != 3 && argc != 4) return 2;
-  const bool mixed = argc == 4;  // "mixed": 32-bit face-record offsets, "mixedwide": 64-bit
-  const bool wide = mixed && std::string(argv[3]) == "mixedwide";
+  const std::string mode = argc == 4 ? argv[3] : "";
+  const bool face = mode == "face";
+  const bool mixed = argc == 4 && !face;  // "mixed": 32-bit face-record offsets, "mixedwide": 64-bit
+  const bool wide = mixed && mode == "mixedwide";
   FILE *f = fopen(argv[1], "rb");
   if (!f) return 3;
   long long nt = 0, n = 0;
@@ -60,6 +66,23 @@ int main(int argc, char **argv) {
         fread(frec.data(), sizeof(sdtfe::FaceRec), 4 * (size_t)nt, f) != 4 * (size_t)nt ||
         fread(org_span, 8, 6, f) != 6)
       return 8;
+  }
+  std::vector<sdtfe::FRec64> frec64;
+  if (face) {
+    long long np = 0;
+    if (fread(&np, 8, 1, f) != 1) return 10;
+    std::vector<double4> pos((size_t)np);
+    std::vector<int> tets(8 * (size_t)nt);
+    if (fread(pos.data(), sizeof(double4), (size_t)np, f) != (size_t)np ||
+        fread(tets.data(), 4, 8 * (size_t)nt, f) != 8 * (size_t)nt)
+      return 11;
+    std::vector<double> nabla(3 * (size_t)nt);
+    for (long long t = 0; t < nt; ++t)
+      for (int k = 0; k < 3; ++k) nabla[3 * (size_t)t + k] = rec[(size_t)t].g[k];
+    frec64.resize(4 * (size_t)nt);
+    for (long long i = 0; i < 4 * nt; ++i)
+      sdtfe::make_frec64(pos.data(), tets.data(), nabla.data(), (int)(i >> 2), (int)(i & 3),
+                         frec64[(size_t)i]);
   }
   fclose(f);
   sdtfe::MapGeom g{};
@@ -108,6 +131,21 @@ int main(int argc, char **argv) {
         v = mc.flt.full_slab ? sdtfe::walk_mixed_resume<true, false>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, key, crossed, resumed, c)
                              : sdtfe::walk_mixed_resume<false, false>(r32, r64, fr, faces, e, x[i], y[i], g, mc.flt, key, crossed, resumed, c);
       if (resumed) c.fp64_redo++;
+    } else if (face) {
+      // the face record the ray enters by: the entry tet's face whose three
+      // sides are positive (as the forward-hull triangle names it on the
+      // GPU); any other face hands the ray to the walk over Rec64 at once
+      const int e = (int)ent[i];
+      int rid = e < 0 ? -1 : 4 * e;
+      for (int fc = 0; e >= 0 && fc < 4; ++fc) {
+        const sdtfe::FRec64 &q = frec64[4 * (size_t)e + fc];
+        const double p2 = g.lz, p3 = -g.lz * y[i], p4 = g.lz * x[i];
+        const double e0 = sdtfe::edge_u(p2, p3, p4, q.p[6], q.p[7], q.p[3], q.p[4]);
+        const double e1 = sdtfe::edge_u(p2, p3, p4, q.p[0], q.p[1], q.p[6], q.p[7]);
+        const double e2 = sdtfe::edge_u(p2, p3, p4, q.p[3], q.p[4], q.p[0], q.p[1]);
+        if (e0 > sdtfe::kZero && e1 > sdtfe::kZero && e2 > sdtfe::kZero) rid = 4 * e + fc;
+      }
+      v = sdtfe::walk64f(rec.data(), frec64.data(), rid, x[i], y[i], g, key, crossed, c);
     } else {
       v = sdtfe::walk64(rec.data(), (int)ent[i], x[i], y[i], g, key, crossed, c);
     }
@@ -118,5 +156,6 @@ int main(int argc, char **argv) {
   }
   fclose(o);
   if (mixed) fprintf(stderr, "fp64_redo %lld full_slab %d\n", redo, mc.flt.full_slab);
+  if (face) fprintf(stderr, "face walk\n");
   return 0;
 }

@@ -231,6 +231,36 @@ def test_sample_split_maps_match_oracle(oracle, S):
     assert _rel(r32, rho) <= 1e-5
 
 
+@pytest.mark.parametrize("S,box_len_z,max_steps", [(1, 1.0, 0), (5, 1.0, 0), (20, 1.0, 0),
+                                                   (1, 0.4, 0), (5, 0.4, 0), (20, 0.4, 0),
+                                                   (5, 1.0, 40)])
+def test_fp64_walks_agree(monkeypatch, S, box_len_z, max_steps):
+    """The FP64 walk over face records (the default) and the walk over
+    per-tet records (SDTFE_FP64_WALK=rec) give the same maps, n_avg and
+    counters bit for bit, with 1, 4 and 16 lanes per pixel, over the whole
+    hull, through a slab that cuts it, and with a step cap that stops some
+    rays."""
+    _say(f"[gpu] clustered-20000 64x64x{S} box_len_z {box_len_z} max_steps {max_steps}: "
+         "face vs rec walk")
+    pos = workloads.clustered(20000, seed=8)
+    tetra = triangulate(pos)
+    pd = to_particle_data(pos)
+    out = {}
+    with capi.Tessellation(pd, tetra) as t:
+        for w in ("face", "rec"):
+            monkeypatch.setenv("SDTFE_FP64_WALK", w)
+            out[w] = t.map(64, S, want_navg=True, seed=3, box_len_z=box_len_z,
+                           max_steps=max_steps)
+    (a, sa, na), (b, sb, nb) = out["face"], out["rec"]
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(na, nb)
+    counters = [k for k in sa if not k.startswith("t_")]
+    assert {k: sa[k] for k in counters} == {k: sb[k] for k in counters}
+    assert sa["tet_steps"] > 0
+    if max_steps:
+        assert sa["step_cap"] > 0
+
+
 def test_lattice_map_matches_oracle(oracle):
     """The large-map layout (jittered lattice plus halos, slab of half the
     box height) at 32x32x16 particles: FP64 map, n_avg and tets against the

@@ -96,7 +96,7 @@ def face_records(pos, tetra, rho):
     return rec.reshape(-1)
 
 
-def run_walk(host_walk, tmp_path, g, mixed=False, wide=False):
+def run_walk(host_walk, tmp_path, g, mixed=False, wide=False, face=False):
     rec = records(g["pos"], g["tetra"], g["rho"], g["nabla"])
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
@@ -116,7 +116,14 @@ def run_walk(host_walk, tmp_path, g, mixed=False, wide=False):
             face_records(g["pos"], g["tetra"], g["rho"]).tofile(f)
             zr = (lo[2], hi[2])
             np.array([*org, float((0.5 * (hi - lo)).max()), *zr]).tofile(f)
-    mode = (["mixedwide" if wide else "mixed"] if mixed else [])
+        if face:
+            pos4 = np.zeros((g["pos"].shape[0], 4))
+            pos4[:, :3] = g["pos"]
+            pos4[:, 3] = g["rho"]
+            np.array([pos4.shape[0]], dtype=np.int64).tofile(f)
+            pos4.tofile(f)
+            np.ascontiguousarray(g["tetra"], dtype=np.int32).tofile(f)
+    mode = (["mixedwide" if wide else "mixed"] if mixed else ["face"] if face else [])
     r = subprocess.run([host_walk, str(inp), str(out)] + mode,
                        check=True, capture_output=True, text=True)
     res = np.fromfile(out).reshape(-1, 2)
@@ -128,6 +135,26 @@ def test_device_walk_code_matches_reference(golden, host_walk, tmp_path):
     res, _ = run_walk(host_walk, tmp_path, g)
     np.testing.assert_array_equal(res[:, 1].astype(np.int32), g["crossed"])
     np.testing.assert_array_equal(res[:, 0], g["value"])
+
+
+@pytest.mark.parametrize("slab", ["hull", "cut"])
+def test_face_record_walk_matches_reference(golden, host_walk, tmp_path, slab):
+    """The FP64 walk over face records (the map kernels' FP64 form) gives
+    the reference walk's values and tets crossed bit for bit: on the golden
+    rays (16,384 stream rays and 600 rays exactly through projected vertices
+    and along projected edges, which take the hand-over to the full face
+    test), and through a slab that cuts the hull."""
+    g = dict(golden("g2_rays.npz"))
+    if slab == "cut":
+        g["lz"], g["off_z"] = 0.5, 0.25
+        want, _ = run_walk(host_walk, tmp_path, g)
+        ref, crossed = want[:, 0], want[:, 1].astype(np.int32)
+    else:
+        ref, crossed = g["value"], g["crossed"]
+    res, err = run_walk(host_walk, tmp_path, g, face=True)
+    assert "face walk" in err
+    np.testing.assert_array_equal(res[:, 1].astype(np.int32), crossed)
+    np.testing.assert_array_equal(res[:, 0], ref)
 
 
 def test_face_table(host_walk):
