@@ -218,19 +218,25 @@ SDTFE_HD inline void load_frec(const FRec64 *__restrict__ fr, int rid, FRec64 &r
   r.code = (unsigned)w.w;
 }
 
-// walk64 over face records, from the record of the entry tet seen from the
-// forward-hull face the ray enters by (4 x tet + face): same value, tets
-// crossed and counters.  The walk over Rec64 takes over at the tet where the
-// fast step cannot decide, with the sum so far (the entry tet itself when
-// the ray does not enter by that face without a special case).
-SDTFE_HD inline double walk64f(const Rec64 *__restrict__ rec, const FRec64 *__restrict__ frec,
-                               int rid, double x, double y, const MapGeom &g, uint64_t key,
-                               int &crossed, LaneCounters &cnt) {
-  crossed = 0;
-  if (rid < 0) return 0.0;
-  const double p2 = g.lz, p3 = -g.lz * y, p4 = g.lz * x;
+// Where the face walk leaves a ray to the walk over Rec64: the tet it
+// could not decide (-1: none, the walk ended on the hull), the tet the ray
+// came from and the tets entered before it.
+struct FaceHand {
+  int t, prev, steps;
+};
+
+// The walk over face records from record rid (4 x tet + face: the entry
+// tet seen from the forward-hull face the ray enters by) along the vertical
+// ray through (x, y), lz long: act(r, z_in, z_out) for each tet crossed, in
+// order, with r holding the tet's vertex A, rho_a and gradient.  Returns
+// the tets crossed; hand says where the walk over Rec64 must take over (the
+// entry tet itself when the ray does not enter by that face without a
+// special case).
+template <class Act>
+SDTFE_HD inline int face_walk(const FRec64 *__restrict__ frec, int rid, double x, double y,
+                              double lz, int max_steps, Act &act, FaceHand &hand) {
+  const double p2 = lz, p3 = -lz * y, p4 = lz * x;
   int prev = rid >> 2, steps = 0, n_cross = 0, hand_t = -1;
-  double acc = 0.0;
   double c0 = 0.0, c1 = 0.0, c2 = 0.0;  // the previous leave face's sides
   bool fresh = true;
   Tet64 r;
@@ -239,13 +245,13 @@ SDTFE_HD inline double walk64f(const Rec64 *__restrict__ rec, const FRec64 *__re
     FRec64 f;
     load_frec(frec, rid, f);
     const unsigned code = f.code;
-    bool hand = steps >= g.max_steps || (code & kFRecHand) != 0;
+    bool hand_here = steps >= max_steps || (code & kFRecHand) != 0;
     double e0, e1, e2;
     if (fresh || (code & kFRecFresh)) {
       e0 = edge_u(p2, p3, p4, f.p[6], f.p[7], f.p[3], f.p[4]);  // u(P2, P1)
       e1 = edge_u(p2, p3, p4, f.p[0], f.p[1], f.p[6], f.p[7]);  // u(P0, P2)
       e2 = edge_u(p2, p3, p4, f.p[3], f.p[4], f.p[0], f.p[1]);  // u(P1, P0)
-      hand = hand || !(e0 > kZero && e1 > kZero && e2 > kZero);
+      hand_here = hand_here || !(e0 > kZero && e1 > kZero && e2 > kZero);
     } else {
       e0 = -sel3(sel_of((int)(code >> 14) & 3), c0, c1, c2);
       e1 = -sel3(sel_of((int)(code >> 16) & 3), c0, c1, c2);
@@ -260,8 +266,8 @@ SDTFE_HD inline double walk64f(const Rec64 *__restrict__ rec, const FRec64 *__re
     const int k = (l1 ? 1 : 0) | (l2 ? 2 : 0);
     const Sel sk = sel_of(k);
     const int nxt = sel3i(sk, f.next[0], f.next[1], f.next[2]);
-    hand = hand || !(l0 || l1 || l2) || nxt == -2 || (nxt >= 0 && (nxt >> 2) == prev);
-    if (hand) {
+    hand_here = hand_here || !(l0 || l1 || l2) || nxt == -2 || (nxt >= 0 && (nxt >> 2) == prev);
+    if (hand_here) {
       hand_t = t;
       break;
     }
@@ -286,7 +292,7 @@ SDTFE_HD inline double walk64f(const Rec64 *__restrict__ rec, const FRec64 *__re
     r.gx = f.g[0];
     r.gy = f.g[1];
     r.gz = f.g[2];
-    acc += tet_integral64(r, x, y, z_in, z_out, g.off_z, g.lz);
+    act(r, z_in, z_out);
     ++n_cross;
     if (nxt < 0) break;
     c0 = g0;
@@ -296,13 +302,31 @@ SDTFE_HD inline double walk64f(const Rec64 *__restrict__ rec, const FRec64 *__re
     prev = t;
     rid = nxt;
   }
-  if (hand_t >= 0) {
+  hand = FaceHand{hand_t, prev, steps};
+  return n_cross;
+}
+
+// walk64 over face records from record rid (see face_walk): same value,
+// tets crossed and counters.  The walk over Rec64 takes over where the face
+// walk cannot decide, with the sum so far.
+SDTFE_HD inline double walk64f(const Rec64 *__restrict__ rec, const FRec64 *__restrict__ frec,
+                               int rid, double x, double y, const MapGeom &g, uint64_t key,
+                               int &crossed, LaneCounters &cnt) {
+  crossed = 0;
+  if (rid < 0) return 0.0;
+  double acc = 0.0;
+  auto integrate = [&](const Tet64 &r, double z_in, double z_out) {
+    acc += tet_integral64(r, x, y, z_in, z_out, g.off_z, g.lz);
+  };
+  FaceHand hand;
+  const int n = face_walk(frec, rid, x, y, g.lz, g.max_steps, integrate, hand);
+  if (hand.t >= 0) {
     int c;
-    const double v = walk64_from(rec, hand_t, prev, steps, x, y, g, key, c, cnt, acc);
-    crossed = c ? n_cross + c : 0;
+    const double v = walk64_from(rec, hand.t, hand.prev, hand.steps, x, y, g, key, c, cnt, acc);
+    crossed = c ? n + c : 0;
     return v;
   }
-  crossed = n_cross;
+  crossed = n;
   return acc;
 }
 
