@@ -646,10 +646,10 @@ def main():
         err = dict(max=float(e.max()), p99=float(np.quantile(e, 0.99)),
                    median=float(np.median(e)))
         log(f"[bench] mixed vs fp64 per-pixel rel err: {err}")
-    # headline form: mixed when it meets 1e-5 against the FP64 map, else FP64
-    head = "mixed" if (err is not None and err["max"] <= 1e-5) else "fp64"
-    if head not in forms:
-        head = next(iter(forms))
+    # headline form: the faster of FP64 (the reference's arithmetic) and the
+    # mixed walk when that meets 1e-5 against the FP64 map
+    ok = [k for k in forms if k == "fp64" or (err is not None and err["max"] <= 1e-5)]
+    head = min(ok or list(forms), key=lambda k: forms[k]["ms_per_step"])
     f = forms[head]
     rays = G * G * S
     value = rays / (f["ms_per_step"] / 1e3)
