@@ -5,11 +5,14 @@
 // perturbation and midpoint integrand follow src/dtfe.c:19-374 operation by
 // operation, and this file is built with -ffp-contract=off.
 //
-// Layout: each step of a walk reads one 144 B record (sdtfe::Rec64) that
-// holds the tet's neighbours, its four vertices and its field, so a step is a
-// single dependent load.  A map launch spreads each pixel's samples over 4
-// lanes (4 or more samples), so the 64 rays of a wave start together on a
-// 4x4-pixel patch of the map and cross mostly the same tets.
+// Layout: each step of a map's walk reads one 144 B face record
+// (sdtfe::FRec64: the entered face's vertices, the fourth vertex, the tet's
+// field and the three ways out), so a step is a single dependent load; the
+// walk over per-tet records (sdtfe::Rec64, also 144 B) finishes what the
+// face walk hands over.  A map launch spreads each pixel's samples over 16
+// lanes (16-64 samples) or 4 (4-15 or 65-256), so the 64 rays of a wave start
+// together on a 2x2- or 4x4-pixel patch of the map and cross mostly the same
+// tets.
 #include <cstdlib>
 #include <string>
 

@@ -1,5 +1,7 @@
 """The map at the workloads' stated sizes, each against the oracle:
 
+* the clustered map: 1,048,576 particles, 2048^2 pixels x 16 samples, FP64
+  against the oracle on sampled 4-row bands, mixed against FP64 everywhere;
 * the large single map: 8,388,608 particles (256 x 256 x 128 jittered
   lattice plus 5% halos), 56.5M tets (so the face records take 64-bit
   offsets), 4096^2 pixels x 8 samples, over the slab of depth 0.5 about
@@ -136,6 +138,45 @@ def test_large_lattice_map(oracle, large_lattice, lz, cz):
     np.testing.assert_array_equal(n64[sel], navg[sel])
     e = _rel(r64[sel], ref[sel])
     _say(f"  fp64 vs oracle max rel err {e:.2e}, bit-equal {(r64[sel] == ref[sel]).mean():.4f}")
+    assert e <= RTOL_FP64
+
+
+@pytest.mark.timeout(900)
+def test_clustered_1m_map(oracle):
+    """1,048,576 clustered particles (30% uniform, 70% in 200 halos), 2048^2
+    x 16 over the whole box: FP64 pixels and sample counts against the
+    oracle on eight sampled 4-row bands, the mixed map against the FP64 map
+    everywhere with the same tets crossed."""
+    _say("[gpu] clustered 1,048,576 particles, 2048^2 x 16")
+    t0 = time.perf_counter()
+    pos = workloads.clustered(1 << 20, seed=0)
+    tetra = triangulate(pos)
+    _say(f"  {tetra.shape[0]} tets in {time.perf_counter() - t0:.1f} s")
+    G, S, seed = 2048, 16, 0
+    with capi.Tessellation(to_particle_data(pos), tetra) as t:
+        r64, s64, n64 = t.map(G, S, want_navg=True, seed=seed)
+        r32, s32, n32 = t.map(G, S, want_navg=True, seed=seed, precision=capi.PREC_MIXED)
+    _say(f"  fp64 walk {s64['t_walk_ms']:.1f} ms, mixed {s32['t_walk_ms']:.1f} ms, "
+         f"handed over {s32['fp64_redo']}")
+    assert s64["samples"] == G * G * S and s64["step_cap"] == 0
+    np.testing.assert_array_equal(n32, n64)
+    assert s32["tet_steps"] == s64["tet_steps"]
+    e = _rel(r32, r64)
+    _say(f"  mixed vs fp64 max rel err {e:.2e}")
+    assert e <= RTOL_MIXED
+    pd = to_particle_data(pos)
+    nabla, tri = _oracle_field(oracle, pd, tetra)
+    step = (G // 4) // 8
+    ref = np.zeros(G * G)
+    navg = np.zeros(G * G, dtype=np.int32)
+    _, rays, _ = oracle.map_bands(pd, tetra, nabla, tri, G, 1.0, 1.0, (0.5, 0.5, 0.5), S, 0.0,
+                                  seed, band_begin=step // 2, band_step=step,
+                                  n_threads=THREADS, rho=ref, n_avg=navg)
+    sel = oracle.band_rows(G, step // 2, step)
+    np.testing.assert_array_equal(n64[sel], navg[sel])
+    e = _rel(r64[sel], ref[sel])
+    _say(f"  oracle: {rays} sample-rays; fp64 vs oracle max rel err {e:.2e}, bit-equal "
+         f"{(r64[sel] == ref[sel]).mean():.4f}")
     assert e <= RTOL_FP64
 
 
