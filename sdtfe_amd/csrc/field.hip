@@ -263,7 +263,9 @@ __global__ void face_record_kernel(const double4 *pos, const int *tets, int nt, 
     int back = -1;
     for (int k = 3; k >= 0; --k)
       if (tets[8 * (int64_t)n + 4 + k] == t) back = k;
-    unsigned code = back < 0 ? kFaceStop : (unsigned)back << 3;
+    // byte f: 8 x back; byte m (m != f): 8 x the label in n of this tet's
+    // vertex m
+    unsigned code = back < 0 ? kFaceStop : (unsigned)back << (8 * f + 3);
     double zs = 0.0;
     for (int k = 1; k <= 3; ++k) {
       const int v = tets[8 * (int64_t)t + (f ^ k)];
@@ -272,7 +274,7 @@ __global__ void face_record_kernel(const double4 *pos, const int *tets, int nt, 
       for (int m = 3; m >= 0; --m)
         if (tets[8 * (int64_t)n + m] == v) lab = m;
       if (lab < 0 || lab == back) code |= kFaceStop;
-      code |= (unsigned)(lab & 3) << (8 * k + 3);
+      code |= (unsigned)(lab & 3) << (8 * (f ^ k) + 3);
     }
     r.code = code;
     const double4 q = pos[tets[8 * (int64_t)n + (back & 3)]];

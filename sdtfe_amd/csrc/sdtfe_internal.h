@@ -81,12 +81,14 @@ static_assert(sizeof(Rec32) == 96, "Rec32 layout");
 //         lo = x - hi rounded), z minus the mean z of face f's three vertices
 //         ((z1 + z2) + z3) / 3 and the density, in FP32
 //   nb    tet across the face (-1 on the hull)
-//   code  8 x the face's index in nb (nb's vertex opposite it, i.e. the new
-//         vertex) in bits 3-4; 8 x the label in nb of t's vertex f^k in
-//         bits 8k..8k+4 (k = 1, 2, 3), so that the walk, which keeps
-//         labels as 8 x label, relabels a vertex with one bit-field extract
-//         at offset 8 (label ^ f); bit 31 (kFaceStop): the ray stops here
-//         (nb < 0, or the mesh lacks the back link)
+//   code  one byte per label of t, each value at bits 3-4 of its byte:
+//         byte m (m != f) holds 8 x the label in nb of t's vertex m, byte f
+//         holds 8 x the face's index in nb (nb's vertex opposite it, i.e.
+//         the new vertex), so that the walk, which keeps labels as
+//         8 x label, relabels a vertex, and finds the new vertex's label,
+//         with one bit-field extract at offset 8 x label; bit 31
+//         (kFaceStop): the ray stops here (nb < 0, or the mesh lacks the
+//         back link)
 struct alignas(16) FaceRec {
   float qxh, qxl, qyh, qyl;
   int nb;

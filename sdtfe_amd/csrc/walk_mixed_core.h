@@ -164,6 +164,17 @@ SDTFE_HD inline FaceRec load_facerec(const FaceRec *__restrict__ frec, int t, in
   return r;
 }
 
+// The byte of a face record's code at offset lab8 = 8 x label (one
+// v_bfe_u32 on the device): 8 x that vertex's label in the next tet, or
+// for the exit face's own label 8 x the face the ray enters the next tet by.
+SDTFE_HD inline int code_field(unsigned code, int lab8) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (int)__builtin_amdgcn_ubfe(code, (unsigned)lab8, 5u);
+#else
+  return (int)((code >> lab8) & 31u);
+#endif
+}
+
 // Vertex densities of tet t (FP32, from the mixed record).
 struct Rho4 {
   float v[4];
@@ -448,15 +459,15 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     p2.z -= dz;
 
     // enter the next tet through its face `back`; its fourth vertex Q
-    const int back8 = (int)(R.code & 0x18u);
+    const int back8 = code_field(R.code, f8);
     // offsets from the FP32 pairs: hi - hi rounds once relative to the
     // offset itself, lo - lo is far below it
     Slot qv{(R.qxh - xh) + (R.qxl - xl), (R.qyh - yh) + (R.qyl - yl),
             fmaf((p0.z + p1.z) + p2.z, 1.0f / 3.0f, R.dqz), R.rho, back8};
     // labels of the carried vertices in the next tet
-    p0.lab8 = (int)((R.code >> (p0.lab8 ^ f8)) & 31u);
-    p1.lab8 = (int)((R.code >> (p1.lab8 ^ f8)) & 31u);
-    p2.lab8 = (int)((R.code >> (p2.lab8 ^ f8)) & 31u);
+    p0.lab8 = code_field(R.code, p0.lab8);
+    p1.lab8 = code_field(R.code, p1.lab8);
+    p2.lab8 = code_field(R.code, p2.lab8);
     t = R.nb;
     enter = back8 >> 3;
 

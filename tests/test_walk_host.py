@@ -54,11 +54,11 @@ def records32(pos, tetra, rho):
 
 def face_records(pos, tetra, rho):
     """The mixed walk's 32-byte face records (4 per tet): the new vertex
-    (x, y as FP32 hi/lo pairs), the tet across face f, a code (8 x the face's
-    index there in bits 3-4, 8 x the labels there of this tet's vertices f^k
-    in bits 8k..8k+4, bit 31 when the walk stops: hull or missing back
-    link), and the new vertex's z relative to the mean z of face f's
-    vertices and its density (FP32)."""
+    (x, y as FP32 hi/lo pairs), the tet across face f, a code (one byte per
+    label m of this tet, value at bits 3-4: 8 x the label there of vertex m,
+    or for m = f 8 x the face's index there; bit 31 when the walk stops: hull
+    or missing back link), and the new vertex's z relative to the mean z of
+    face f's vertices and its density (FP32)."""
     t = tetra.shape[0]
     rec = np.zeros((t, 4), dtype=np.dtype([("qxh", "<f4"), ("qxl", "<f4"), ("qyh", "<f4"),
                                            ("qyl", "<f4"), ("nb", "<i4"), ("code", "<u4"),
@@ -74,7 +74,7 @@ def face_records(pos, tetra, rho):
         back = np.full(t, -1)
         for k in range(3, -1, -1):
             back = np.where(nbrs[nn, k] == np.arange(t), k, back)
-        code = np.where(back < 0, stop, (back & 3) << 3).astype(np.int64)
+        code = np.where(back < 0, stop, (back & 3) << (8 * f + 3)).astype(np.int64)
         zs = np.zeros(t)
         for k in (1, 2, 3):
             v = verts[:, f ^ k]
@@ -83,7 +83,7 @@ def face_records(pos, tetra, rho):
             for m in range(3, -1, -1):
                 lab = np.where(verts[nn, m] == v, m, lab)
             code |= np.where((lab < 0) | (lab == back), stop, 0)
-            code |= (lab & 3) << (8 * k + 3)
+            code |= (lab & 3) << (8 * (f ^ k) + 3)
         q = verts[nn, back & 3]
         rec["code"][:, f] = np.where(ok, code, stop)
         for c, k in ((0, "qx"), (1, "qy")):
