@@ -714,7 +714,10 @@ void compute_density(double *particle_data, int n_particles, int *tetra_data, in
                      double center_x, double center_y, double center_z, const int n_mc_samp,
                      const double delta_sample) {
   sdtfe_tess *t = nullptr;
-  int rc = sdtfe_tess_create(particle_data, n_particles, tetra_data, n_tetra, p_mass, &t);
+  const int precision = dropin_precision();
+  // a mixed map never walks the FP64 face records: they are not built
+  int rc = tess_create_staged(particle_data, n_particles, tetra_data, n_tetra, p_mass, nullptr,
+                              nullptr, precision == SDTFE_PREC_FP64, &t);
   if (rc == 0) {
     sdtfe_map_params p{};
     p.grid_dim = grid_dim;
@@ -726,7 +729,7 @@ void compute_density(double *particle_data, int n_particles, int *tetra_data, in
     p.center[2] = center_z;
     p.delta_sample = delta_sample;
     p.seed = g_seed.fetch_add(1);
-    p.precision = dropin_precision();
+    p.precision = precision;
     rc = sdtfe_map(t, &p, rho, nullptr, nullptr);
     if (rc == 0 && n_particles > 0) {
       std::vector<double> r((size_t)n_particles);
