@@ -405,7 +405,9 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
   // kCrossRound acc and is added at the end.  Crossing 0 is the FP64 one.
   float E = 0.f, eps_p = 0.f, erho_p = 0.f, a_p = 0.f;
   FaceRec R = load_facerec<kWide>(frec, t, f8);
-  for (;;) {
+  // one step: false at the walk's exit; two per iteration, so that the
+  // values one step hands the next alternate between registers
+  auto step = [&]() -> bool {
     // exit crossing in FP32, height above the entry crossing (does not wait
     // for R)
     const float den = (e0 + e1) + e2;
@@ -427,7 +429,7 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
       r_out = o.rho;
       q = 0.f;
     }
-    if (stop) break;
+    if (stop) return false;
     bool ok = true;
     bool inside = true;
     if (kFullSlab) {
@@ -517,6 +519,11 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     if (!kFullSlab) slab_fail = slab_fail || !ok;
     const bool fail = zero || !(r0 || r1 || r2) || !ok;
     steps = fail ? ~steps : steps + 1;
+    return true;
+  };
+  for (;;) {
+    if (!step()) break;
+    if (!step()) break;
   }
   if (steps < 0) {
     // t is the tet just entered (a midpoint left open: the whole ray)
