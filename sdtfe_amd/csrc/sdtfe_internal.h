@@ -204,15 +204,18 @@ __device__ inline void add_stat(unsigned long long *stats, int idx, unsigned lon
 
 // --------------------------------------------------------- launch helpers
 
-// Blocks are dealt to the 8 XCDs round-robin; remap so that consecutive
-// logical tiles (neighbouring pixels, which walk the same tets) land on the
-// same XCD and share its L2.
-__device__ inline int xcd_tile(int block, int n_blocks) {
-  const int n_xcd = 8;
-  int per = n_blocks / n_xcd;
-  int full = per * n_xcd;
+// Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD takes
+// whole rows of `row` consecutive tiles in turn (row r on XCD r mod 8):
+// neighbouring tiles of a row, which walk the same tets, share an L2, and
+// each XCD's rows spread over the whole map, so that its share of the work
+// follows the map's mean density (contiguous eighths of the map left some
+// XCDs with more of the dense halos than others).
+__device__ inline int xcd_tile(int block, int n_blocks, int row) {
+  const int n_xcd = 8, group = n_xcd * row;
+  const int full = row > 0 ? (n_blocks / group) * group : 0;
   if (block >= full) return block;
-  return (block % n_xcd) * per + block / n_xcd;
+  const int x = block % n_xcd, k = block / n_xcd;
+  return (k / row) * group + x * row + k % row;
 }
 
 }  // namespace sdtfe

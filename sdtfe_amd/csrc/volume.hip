@@ -282,7 +282,7 @@ SDTFE_HD inline void walk_column(const Rec64 *__restrict__ rec, const Locate &lo
 
 // 16 x 16 positions per block, an 8 x 8 patch per wave.
 __device__ inline void block_patch(int tiles_y, int &px, int &py) {
-  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int b = xcd_tile((int)blockIdx.x, (int)gridDim.x, tiles_y);
   const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
   px = (b / tiles_y) * 16 + (w >> 1) * 8 + (lane >> 3);
   py = (b % tiles_y) * 16 + (w & 1) * 8 + (lane & 7);

@@ -115,12 +115,12 @@ SDTFE_HD constexpr int map_window_words(int L) { return L == 4 ? 4 : 1; }
 
 // Pixel and sample slice j (0..L-1) of a lane with L lanes per pixel:
 // blocks of 256 lanes cover 2 x 2 wave tiles; block ids are remapped so
-// neighbouring tiles share an XCD.
+// that each XCD takes whole rows of tiles in turn.
 template <int L>
 __device__ inline void split_pixel(int grid, int &px, int &py, int &j) {
   constexpr int bw = wave_bw(L), bh = wave_bh(L);
   const int nbx = (grid + 2 * bw - 1) / (2 * bw);
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x, nbx);
   const int bx = tile % nbx, by = tile / nbx;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int q = lane / L;

@@ -240,7 +240,9 @@ SDTFE_HD inline int face_walk(const FRec64 *__restrict__ frec, int rid, double x
   double c0 = 0.0, c1 = 0.0, c2 = 0.0;  // the previous leave face's sides
   bool fresh = true;
   Tet64 r;
-  for (;;) {
+  // one step: false at the walk's end or a hand-over; two per iteration,
+  // so that the values one step hands the next alternate between registers
+  auto step = [&]() -> bool {
     const int t = rid >> 2;
     FRec64 f;
     load_frec(frec, rid, f);
@@ -269,7 +271,7 @@ SDTFE_HD inline int face_walk(const FRec64 *__restrict__ frec, int rid, double x
     hand_here = hand_here || !(l0 || l1 || l2) || nxt == -2 || (nxt >= 0 && (nxt >> 2) == prev);
     if (hand_here) {
       hand_t = t;
-      break;
+      return false;
     }
     ++steps;
     const double z_in = face_z(e0, e1, e2, sel_of((int)(code & 3)), f.p[2], f.p[5], f.p[8]);
@@ -294,13 +296,18 @@ SDTFE_HD inline int face_walk(const FRec64 *__restrict__ frec, int rid, double x
     r.gz = f.g[2];
     act(r, z_in, z_out);
     ++n_cross;
-    if (nxt < 0) break;
+    if (nxt < 0) return false;
     c0 = g0;
     c1 = g1;
     c2 = g2;
     fresh = false;
     prev = t;
     rid = nxt;
+    return true;
+  };
+  for (;;) {
+    if (!step()) break;
+    if (!step()) break;
   }
   hand = FaceHand{hand_t, prev, steps};
   return n_cross;
