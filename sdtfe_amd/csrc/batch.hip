@@ -206,10 +206,11 @@ __device__ __forceinline__ void batch_body(const BatchItem *__restrict__ items,
   }
 }
 
-__global__ __launch_bounds__(256) void batch_kernel_fp64(const BatchItem *__restrict__ items,
-                                                         const BatchHdr *__restrict__ hdr,
-                                                         int *counters,
-                                                         unsigned long long *item_stats) {
+// Built for 3 waves per SIMD, as the single-map FP64 kernel is (left to
+// itself the allocator takes 178 VGPRs, which allows only 2).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void batch_kernel_fp64(
+    const BatchItem *__restrict__ items, const BatchHdr *__restrict__ hdr, int *counters,
+    unsigned long long *item_stats) {
   extern __shared__ double lds[];
   __shared__ int s_tile;
   batch_body<false, false>(items, hdr, counters, item_stats, lds, s_tile, nullptr);
