@@ -525,14 +525,26 @@ SDTFE_HD inline bool walk_mixed(const Rec32 *__restrict__ rec32, const Rec64 *__
     if (!step()) break;
     if (!step()) break;
   }
+  // A hand-over in mid-ray keeps the mixed prefix acc when its error bound
+  // is within tol of it: the crossings' terms so far (E), the last
+  // crossing's height error against the last chord (it does not cancel: the
+  // FP64 walk forms that crossing again for the next chord), and the
+  // densities' rounding.  The ray's value is at least its prefix's (the
+  // integrand is not negative), so the whole ray then keeps the bound;
+  // otherwise the FP64 walk takes the whole ray.
+  auto resume_mid = [&](int before) {
+    const float bound = fmaf(eps_p, a_p, fmaf(kCrossRound, (float)acc, E));
+    return bound <= flt.tol * (float)acc ? Resume{t, enter, before, acc}
+                                         : Resume{entry, -1, 0, 0.0};
+  };
   if (steps < 0) {
     // t is the tet just entered (a midpoint left open: the whole ray)
-    rs = (!kFullSlab && slab_fail) ? Resume{entry, -1, 0, 0.0} : Resume{t, enter, ~steps, acc};
+    rs = (!kFullSlab && slab_fail) ? Resume{entry, -1, 0, 0.0} : resume_mid(~steps);
     return false;
   }
   if ((int)R.code < 0) {
     if (R.nb >= 0) {  // the mesh lacks the back link: the FP64 walk from t
-      rs = Resume{t, enter, steps - 1, acc};
+      rs = resume_mid(steps - 1);
       return false;
     }
     // the last crossing in FP64, where the lanes of a wave are together again
