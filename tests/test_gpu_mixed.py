@@ -77,6 +77,31 @@ def test_mixed_map_matches_fp64(kind, n, G, S, box_z, cz):
     assert e.max() <= RTOL_MIXED
 
 
+@pytest.mark.parametrize("S,box_z,cz", [(8, 1.0, 0.5), (16, 1.0, 0.5), (4, 0.3, 0.45)])
+def test_zero_tolerance_gives_the_fp64_map(S, box_z, cz):
+    """With the per-ray tolerance at 0 no FP32 value can stand: a ray handed
+    over at its end (its bound) and a ray handed over in mid-ray (a side the
+    filter leaves open, whose mixed prefix must then meet the tolerance too)
+    both go to the FP64 walk from their entry, so the mixed map equals the
+    FP64 map bit for bit, sample counts and tets crossed included."""
+    _say(f"[gpu] mixed map at tolerance 0, {S} samples, slab {box_z}")
+    pos = workloads.clustered(65536, seed=11)
+    tetra = triangulate(pos)
+    kw = dict(seed=4, box_len_z=box_z, center=(0.5, 0.5, cz))
+    with capi.Tessellation(to_particle_data(pos), tetra) as t:
+        r64, s64, n64 = t.map(256, S, want_navg=True, **kw)
+        os.environ["SDTFE_MIXED_TOL"] = "0"
+        try:
+            r32, s32, n32 = t.map(256, S, want_navg=True, precision=capi.PREC_MIXED, **kw)
+        finally:
+            os.environ.pop("SDTFE_MIXED_TOL", None)
+    _say(f"  handed over {s32['fp64_redo']} of {s32['samples']}")
+    assert s32["fp64_redo"] > 0.5 * s32["samples"]
+    np.testing.assert_array_equal(n32, n64)
+    assert s32["tet_steps"] == s64["tet_steps"]
+    np.testing.assert_array_equal(r32, r64)
+
+
 def test_mixed_map_is_deterministic():
     """Two runs of a mixed map agree bit for bit: each pixel has one writer
     and a fixed summation order, also where samples go to the FP64 pass."""
