@@ -485,8 +485,12 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
   float ms = 0;
   SDTFE_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   std::vector<unsigned long long> st((size_t)n * ST_COUNT);
-  SDTFE_HIP(hipMemcpy(st.data(), b->d_stats, sizeof(unsigned long long) * st.size(),
-                      hipMemcpyDeviceToHost));
+  // on the batch's stream: the null stream would order the copy against
+  // other work of the process (a server's set-up threads issue theirs
+  // meanwhile)
+  SDTFE_HIP(hipMemcpyAsync(st.data(), b->d_stats, sizeof(unsigned long long) * st.size(),
+                           hipMemcpyDeviceToHost, b->stream));
+  SDTFE_HIP(hipStreamSynchronize(b->stream));
   sdtfe_stats sum{};
   for (int i = 0; i < n; ++i) {
     sdtfe_stats s{};

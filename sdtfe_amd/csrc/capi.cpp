@@ -295,9 +295,13 @@ void free_tess(sdtfe_tess *t) {
                   t->cell_tri, t->stats,   t->map_buf, t->navg_buf, t->defer_buf,
                   t->defer_count, t->dq_off, t->dq_tmp, t->dq_val, t->dq_ok};
   if (t->stream) {
-    (void)hipStreamSynchronize(t->stream);
+    // The frees are stream-ordered: work still queued on the stream ends
+    // first.  A stream of the caller's pool is not drained, so freeing never
+    // waits for other work the caller has queued on it; its owner has
+    // finished any work on the tessellation issued on other streams.
+    if (t->own_stream) (void)hipStreamSynchronize(t->stream);
     for (void *b : bufs) dev_free(b, t->stream);
-    (void)hipStreamSynchronize(t->stream);
+    if (t->own_stream) (void)hipStreamSynchronize(t->stream);
   }
   if (t->ev0) (void)hipEventDestroy(t->ev0);
   if (t->ev1) (void)hipEventDestroy(t->ev1);

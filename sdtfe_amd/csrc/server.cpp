@@ -7,16 +7,21 @@
 //                 footprint fits the budget and fewer than max_active are
 //                 resident, upload them and build their tessellations
 //                 (sdtfe_tess_create), several requests at a time (uploads
-//                 overlap each other's kernels).  Requests take their
-//                 streams round-robin from a pool of twice the setup threads
-//                 (at most 16), so the streams a server holds stay bounded
-//                 however many requests are resident.
+//                 overlap each other's kernels).  Each thread sets its
+//                 requests up on two streams of a pool of twice the setup
+//                 threads (at most 16) in turn, so the streams a server holds
+//                 stay bounded however many requests are resident, and one
+//                 thread's set-up never waits for another's.
 //   walk thread   repeatedly launches one persistent batch over the next
 //                 tiles of every resident request (at most tile_budget tiles
 //                 per launch, shared round-robin), so a request admitted
 //                 while others are on the GPU joins the next launch; a
-//                 request whose tiles are all done is copied out, freed and
-//                 its waiters woken.
+//                 request whose tiles are all done goes to the completion
+//                 thread, and the next launch does not wait for it.
+//   completion    copies a finished map out on a stream of its own (through
+//   thread        pinned memory), adds it into the caller's map, frees the
+//                 request's device memory and wakes its waiters.  Nothing
+//                 there waits for the streams other requests set up on.
 //
 // Every pixel is computed by the same code with the same stream whatever
 // else shares the launch, so a map is identical alone and in a mix.
@@ -57,6 +62,23 @@ struct Request {
   sdtfe_stats stats{};
 };
 
+// A request whose walk is over, for the completion thread: the launch's
+// status and, when it failed, the walk thread's message.
+struct Finished {
+  std::shared_ptr<Request> r;
+  int status = 0;
+  std::string error;
+};
+
+// The completion thread's pinned buffer for copying maps out.
+struct Pinned {
+  double *p = nullptr;
+  size_t n = 0;
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 int64_t estimate_footprint(int n, int nt, int grid, bool fp64) {
   // resident: pos4 32 B/vertex; per tet: row 32, volume 8, gradient 24,
   // records 144 (FP64) + 96 (mixed) + 4 x 32 (faces) and, for FP64 maps,
@@ -71,22 +93,24 @@ int64_t estimate_footprint(int n, int nt, int grid, bool fp64) {
 struct sdtfe_server {
   sdtfe_server_config cfg{};
   std::mutex mu;
-  std::condition_variable cv_setup, cv_walk, cv_done;
+  std::condition_variable cv_setup, cv_walk, cv_done, cv_complete;
   std::deque<std::shared_ptr<sdtfe::Request>> queued;
   std::vector<std::shared_ptr<sdtfe::Request>> active;  // resident, tiles pending
+  std::deque<sdtfe::Finished> completing;                // walked, not yet copied out
   std::map<int64_t, std::shared_ptr<sdtfe::Request>> all;
   int64_t next_id = 1;
   int64_t used_bytes = 0;
-  int n_setup = 0;  // being set up (counted in used_bytes)
+  int n_setup = 0;       // being set up (counted in used_bytes)
+  int n_completing = 0;  // handed to the completion thread, not yet done
   bool stop = false;
   sdtfe_batch *batch = nullptr;
   std::vector<std::thread> setup_thr;
-  std::thread walk_thr;
+  std::thread walk_thr, complete_thr;
+  hipStream_t copy_stream = nullptr;  // the completion thread's
   sdtfe::Clock::time_point t0;
   int64_t launches = 0;
   double walk_ms = 0;
-  std::vector<hipStream_t> streams;  // the pool requests set up and copy out on
-  size_t next_stream = 0;            // round-robin cursor (under mu)
+  std::vector<hipStream_t> streams;  // the pool requests are set up on
 };
 
 namespace sdtfe {
@@ -109,23 +133,38 @@ void add_stats(sdtfe_stats &a, const sdtfe_stats &s) {
   a.fp64_redo += s.fp64_redo;
 }
 
-void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status) {
-  // caller holds no lock; device work of r is complete.  A failing status
-  // comes with this thread's last error, which the waiter cannot see.
-  std::string error = status ? std::string(sdtfe_last_error()) : std::string();
-  if (status == 0 && r->d_map) {
-    // on the request's own stream: a copy on the null stream would wait for
-    // every other request's setup
-    const size_t npix = (size_t)r->params.grid_dim * r->params.grid_dim;
-    std::vector<double> m(npix);
-    if (hipMemcpyAsync(m.data(), r->d_map, sizeof(double) * npix, hipMemcpyDeviceToHost,
-                       r->tess->stream) != hipSuccess ||
-        hipStreamSynchronize(r->tess->stream) != hipSuccess) {
-      status = SDTFE_EHIP;
-      error = "copying the map back failed";
-    } else
-      for (size_t i = 0; i < npix; ++i) r->out[i] += m[i];
+// Adds a walked request's map into the caller's host map: slices of at most
+// 2^20 pixels go through the pinned buffer on the copy stream, which carries
+// nothing else.  The walk that wrote the map has finished.
+int copy_out(sdtfe_server *sv, Request &r, Pinned &pb) {
+  const size_t npix = (size_t)r.params.grid_dim * r.params.grid_dim;
+  const size_t slice = std::min(npix, (size_t)1 << 20);
+  if (pb.n < slice) {
+    if (pb.p) (void)hipHostFree(pb.p);
+    pb.p = nullptr;
+    pb.n = 0;
+    if (hipHostMalloc((void **)&pb.p, sizeof(double) * slice, hipHostMallocDefault) !=
+        hipSuccess) {
+      pb.p = nullptr;
+      return fail(SDTFE_ENOMEM, "pinned copy-out buffer allocation failed");
+    }
+    pb.n = slice;
   }
+  for (size_t off = 0; off < npix; off += slice) {
+    const size_t n = std::min(slice, npix - off);
+    SDTFE_HIP(hipMemcpyAsync(pb.p, r.d_map + off, sizeof(double) * n, hipMemcpyDeviceToHost,
+                             sv->copy_stream));
+    SDTFE_HIP(hipStreamSynchronize(sv->copy_stream));
+    double *out = r.out + off;
+    for (size_t i = 0; i < n; ++i) out[i] += pb.p[i];
+  }
+  return 0;
+}
+
+// Frees a request's device memory and wakes its waiters; status and error
+// are what the waiter sees.  The caller holds no lock, and the request's
+// device work is complete (its frees are ordered on its stream).
+void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status, std::string error) {
   if (r->d_map) dev_free(r->d_map, r->tess->stream);
   r->d_map = nullptr;
   sdtfe_tess_destroy(r->tess);
@@ -142,9 +181,13 @@ void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status) {
   sv->cv_setup.notify_all();
 }
 
-void setup_loop(sdtfe_server *sv) {
+void setup_loop(sdtfe_server *sv, int thread_no) {
   Staging st;  // this thread's pinned upload buffers
   if (staging_init(st, (size_t)8 << 20)) staging_free(st);  // unpinned uploads then
+  // this thread's two streams of the pool, in turn (shared with another
+  // thread only beyond 8 set-up threads)
+  const size_t n_streams = sv->streams.size();
+  size_t turn = 0;
   for (;;) {
     std::shared_ptr<Request> r;
     hipStream_t stream = nullptr;
@@ -154,7 +197,7 @@ void setup_loop(sdtfe_server *sv) {
         if (sv->stop) return true;
         if (sv->queued.empty()) return false;
         const auto &q = sv->queued.front();
-        const int resident = (int)sv->active.size() + sv->n_setup;
+        const int resident = (int)sv->active.size() + sv->n_setup + sv->n_completing;
         const bool fits = sv->used_bytes + q->footprint <= sv->cfg.hbm_budget_bytes ||
                           (resident == 0);  // a lone request is always admitted
         return resident < sv->cfg.max_active && fits;
@@ -169,7 +212,7 @@ void setup_loop(sdtfe_server *sv) {
       sv->used_bytes += r->footprint;
       sv->n_setup++;
       r->t_admit = Clock::now();
-      stream = sv->streams[sv->next_stream++ % sv->streams.size()];
+      stream = sv->streams[(2 * (size_t)thread_no + (turn++ & 1)) % n_streams];
     }
     // a mixed-precision server never walks the FP64 face records
     int rc = tess_create_staged(r->pd.data(), (int)(r->pd.size() / 4), r->tets.data(),
@@ -191,15 +234,18 @@ void setup_loop(sdtfe_server *sv) {
                                            : map_lanes(r->params.n_mc_samp);
       r->n_tiles = batch_tiles(r->params, r->lpp);
     }
-    {
+    if (rc) {
+      finish(sv, r, rc, sdtfe_last_error());
       std::lock_guard<std::mutex> g(sv->mu);
       sv->n_setup--;
-    }
-    if (rc) {
-      finish(sv, r, rc);
+      sv->cv_done.notify_all();
+      sv->cv_setup.notify_all();
       continue;
     }
+    // from being set up to resident in one step: the drain in
+    // sdtfe_server_destroy never sees the request in neither state
     std::lock_guard<std::mutex> g(sv->mu);
+    sv->n_setup--;
     sv->active.push_back(r);
     sv->cv_walk.notify_all();
   }
@@ -288,9 +334,37 @@ void walk_loop(sdtfe_server *sv) {
           ++it;
         }
       }
+      // to the completion thread; a request in no launch does not take the
+      // launch's status
+      const std::string error = rc ? std::string(sdtfe_last_error()) : std::string();
+      for (auto &f : finished) sv->completing.push_back(Finished{f, rc, error});
+      for (auto &f : empty) sv->completing.push_back(Finished{f, 0, std::string()});
+      sv->n_completing += (int)(finished.size() + empty.size());
     }
-    for (auto &f : finished) finish(sv, f, rc);
-    for (auto &f : empty) finish(sv, f, 0);  // in no launch: the launch's status is not theirs
+    if (!finished.empty() || !empty.empty()) sv->cv_complete.notify_one();
+  }
+}
+
+void complete_loop(sdtfe_server *sv) {
+  Pinned pb;
+  for (;;) {
+    Finished f;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_complete.wait(lk, [&] { return sv->stop || !sv->completing.empty(); });
+      if (sv->completing.empty()) return;  // stopped, nothing left
+      f = std::move(sv->completing.front());
+      sv->completing.pop_front();
+    }
+    if (f.status == 0 && f.r->d_map && copy_out(sv, *f.r, pb)) {
+      f.status = SDTFE_EHIP;
+      f.error = std::string("copying the map back failed: ") + sdtfe_last_error();
+    }
+    finish(sv, f.r, f.status, std::move(f.error));
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->n_completing--;
+    sv->cv_done.notify_all();
+    sv->cv_setup.notify_all();
   }
 }
 
@@ -323,7 +397,7 @@ int sdtfe_server_create(const sdtfe_server_config *cfg, sdtfe_server **out) {
   sv->t0 = Clock::now();
   const int n_setup = cfg->setup_threads > 0 ? std::min(cfg->setup_threads, 64) : 4;
   const int n_streams = std::min(2 * n_setup, 16);
-  for (int i = 0; i < n_streams; ++i) {
+  for (int i = 0; i <= n_streams; ++i) {  // the pool, then the copy stream
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
       for (hipStream_t x : sv->streams) (void)hipStreamDestroy(x);
@@ -331,10 +405,14 @@ int sdtfe_server_create(const sdtfe_server_config *cfg, sdtfe_server **out) {
       delete sv;
       return fail(SDTFE_EHIP, "server stream pool");
     }
-    sv->streams.push_back(st);
+    if (i < n_streams)
+      sv->streams.push_back(st);
+    else
+      sv->copy_stream = st;
   }
-  for (int i = 0; i < n_setup; ++i) sv->setup_thr.emplace_back(setup_loop, sv);
+  for (int i = 0; i < n_setup; ++i) sv->setup_thr.emplace_back(setup_loop, sv, i);
   sv->walk_thr = std::thread(walk_loop, sv);
+  sv->complete_thr = std::thread(complete_loop, sv);
   *out = sv;
   return 0;
 }
@@ -398,7 +476,7 @@ int sdtfe_server_get_info(sdtfe_server *sv, sdtfe_server_info *info) {
   if (!sv || !info) return fail(SDTFE_EINVAL, "null argument");
   std::lock_guard<std::mutex> g(sv->mu);
   info->queued = (int)sv->queued.size();
-  info->active = (int)sv->active.size() + sv->n_setup;
+  info->active = (int)sv->active.size() + sv->n_setup + sv->n_completing;
   info->used_bytes = sv->used_bytes;
   info->launches = sv->launches;
   info->walk_ms = sv->walk_ms;
@@ -409,21 +487,29 @@ void sdtfe_server_destroy(sdtfe_server *sv) {
   if (!sv) return;
   {
     std::unique_lock<std::mutex> lk(sv->mu);
-    // drain: wait until nothing is queued, resident or being set up
+    // drain: wait until nothing is queued, being set up, resident or
+    // being copied out
     sv->cv_done.wait(lk, [&] {
-      return sv->queued.empty() && sv->active.empty() && sv->n_setup == 0;
+      return sv->queued.empty() && sv->active.empty() && sv->n_setup == 0 &&
+             sv->n_completing == 0;
     });
     sv->stop = true;
     sv->cv_setup.notify_all();
     sv->cv_walk.notify_all();
+    sv->cv_complete.notify_all();
   }
   for (std::thread &th : sv->setup_thr)
     if (th.joinable()) th.join();
   if (sv->walk_thr.joinable()) sv->walk_thr.join();
+  if (sv->complete_thr.joinable()) sv->complete_thr.join();
   sdtfe_batch_destroy(sv->batch);
   for (hipStream_t st : sv->streams) {
     (void)hipStreamSynchronize(st);
     (void)hipStreamDestroy(st);
+  }
+  if (sv->copy_stream) {
+    (void)hipStreamSynchronize(sv->copy_stream);
+    (void)hipStreamDestroy(sv->copy_stream);
   }
   delete sv;
 }
