@@ -113,6 +113,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const int *in,
 
 size_t scan_tmp_ints(int64_t n) { return (size_t)((n + kScanChunk - 1) / kScanChunk) + 1; }
 
+// Where scan_exclusive leaves the total of n > 0 items in its tmp space.
+const int *scan_total_device(const int *tmp, int64_t n) {
+  return tmp + (n + kScanChunk - 1) / kScanChunk;
+}
+
 int scan_exclusive(const int *in, int *out, int64_t n, int *tmp, int *total_host,
                    hipStream_t s) {
   if (n <= 0) {
@@ -309,25 +314,54 @@ __device__ inline void face_triple(int f, int4 v, int &p0, int &p1, int &p2) {
   }
 }
 
-__global__ void hull_mask_kernel(const double4 *pos, const int *tets, int nt,
-                                 unsigned char *mask, int *cnt) {
+// Forward-hull faces of each tet (bit f: face f has no neighbour and faces
+// down) and their count.  With part, also each block's bounding box of those
+// faces' vertices (x min, y min, x max, y max at part + 4 x block).  Blocks
+// of 256 threads.
+__global__ __launch_bounds__(256) void hull_mask_kernel(const double4 *pos, const int *tets,
+                                                        int nt, unsigned char *mask, int *cnt,
+                                                        double *part) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
-  const int4 v = tet_verts(tets, t), nb = tet_nbrs(tets, t);
-  const int nbf[4] = {nb.x, nb.y, nb.z, nb.w};
-  unsigned m = 0;
+  double lo_x = INFINITY, lo_y = INFINITY, hi_x = -INFINITY, hi_y = -INFINITY;
+  if (t < nt) {
+    const int4 v = tet_verts(tets, t), nb = tet_nbrs(tets, t);
+    const int nbf[4] = {nb.x, nb.y, nb.z, nb.w};
+    unsigned m = 0;
 #pragma unroll
-  for (int f = 0; f < 4; ++f) {
-    if (nbf[f] != -1) continue;
-    int i0, i1, i2;
-    face_triple(f, v, i0, i1, i2);
-    const double4 p0 = pos[i0], p1 = pos[i1], p2 = pos[i2];
-    double t1x = p1.x - p0.x, t1y = p1.y - p0.y;
-    double t2x = p2.x - p0.x, t2y = p2.y - p0.y;
-    if (t1x * t2y - t1y * t2x < 0.0) m |= 1u << f;
+    for (int f = 0; f < 4; ++f) {
+      if (nbf[f] != -1) continue;
+      int i0, i1, i2;
+      face_triple(f, v, i0, i1, i2);
+      const double4 p0 = pos[i0], p1 = pos[i1], p2 = pos[i2];
+      double t1x = p1.x - p0.x, t1y = p1.y - p0.y;
+      double t2x = p2.x - p0.x, t2y = p2.y - p0.y;
+      if (t1x * t2y - t1y * t2x < 0.0) {
+        m |= 1u << f;
+        lo_x = fmin(lo_x, fmin(fmin(p0.x, p1.x), p2.x));
+        lo_y = fmin(lo_y, fmin(fmin(p0.y, p1.y), p2.y));
+        hi_x = fmax(hi_x, fmax(fmax(p0.x, p1.x), p2.x));
+        hi_y = fmax(hi_y, fmax(fmax(p0.y, p1.y), p2.y));
+      }
+    }
+    mask[t] = (unsigned char)m;
+    cnt[t] = __popc(m);
   }
-  mask[t] = (unsigned char)m;
-  cnt[t] = __popc(m);
+  if (!part) return;  // the same for every thread of the launch
+  __shared__ double s[4][256];
+  s[0][threadIdx.x] = lo_x; s[1][threadIdx.x] = lo_y;
+  s[2][threadIdx.x] = hi_x; s[3][threadIdx.x] = hi_y;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s[0][threadIdx.x] = fmin(s[0][threadIdx.x], s[0][threadIdx.x + h]);
+      s[1][threadIdx.x] = fmin(s[1][threadIdx.x], s[1][threadIdx.x + h]);
+      s[2][threadIdx.x] = fmax(s[2][threadIdx.x], s[2][threadIdx.x + h]);
+      s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int j = 0; j < 4; ++j) part[4 * (int64_t)blockIdx.x + j] = s[j][0];
 }
 
 __global__ void emit_tri_kernel(const double4 *pos, const int *tets, int nt,
@@ -355,18 +389,19 @@ __global__ void emit_tri_kernel(const double4 *pos, const int *tets, int nt,
   }
 }
 
-__global__ __launch_bounds__(256) void tri_bbox_kernel(const double *tri_xy, int n_tri,
-                                                       double *out) {
+// The forward hull's bounding box from hull_mask_kernel's nb block boxes,
+// and its triangle count (the scan's total): out = {x min, y min, x max,
+// y max, count}, one read-back for both.  One block of 256 threads.
+__global__ __launch_bounds__(256) void hull_box_kernel(const double *part, int nb,
+                                                       const int *count, double *out) {
   __shared__ double s[4][256];
   double lo_x = INFINITY, lo_y = INFINITY, hi_x = -INFINITY, hi_y = -INFINITY;
-  for (int k = threadIdx.x; k < n_tri; k += blockDim.x) {
-    const double *xy = tri_xy + 6 * (int64_t)k;
-    for (int j = 0; j < 3; ++j) {
-      lo_x = fmin(lo_x, xy[2 * j]);
-      hi_x = fmax(hi_x, xy[2 * j]);
-      lo_y = fmin(lo_y, xy[2 * j + 1]);
-      hi_y = fmax(hi_y, xy[2 * j + 1]);
-    }
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    const double *b = part + 4 * (int64_t)k;
+    lo_x = fmin(lo_x, b[0]);
+    lo_y = fmin(lo_y, b[1]);
+    hi_x = fmax(hi_x, b[2]);
+    hi_y = fmax(hi_y, b[3]);
   }
   s[0][threadIdx.x] = lo_x; s[1][threadIdx.x] = lo_y;
   s[2][threadIdx.x] = hi_x; s[3][threadIdx.x] = hi_y;
@@ -380,8 +415,10 @@ __global__ __launch_bounds__(256) void tri_bbox_kernel(const double *tri_xy, int
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     for (int j = 0; j < 4; ++j) out[j] = s[j][0];
+    out[4] = (double)*count;
+  }
 }
 
 template <bool kFill>
@@ -518,8 +555,8 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra, Staging *st) 
     face_record64_kernel<<<blocks_for(4 * (int64_t)nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, t->nabla, nt, t->frec64);
   }
+  // the entry grid follows on the same stream and ends with its sync
   SDTFE_HIP(hipGetLastError());
-  SDTFE_HIP(hipStreamSynchronize(s));
   return 0;
 }
 
@@ -529,15 +566,30 @@ int build_entry_grid(sdtfe_tess *t) {
   Scratch sc(s);
   unsigned char *mask = nullptr;
   int *cnt = nullptr, *off = nullptr, *tmp = nullptr;
+  const unsigned nblk = blocks_for(nt, 256);
+  double *part = nullptr, *box = nullptr;
   SDTFE_RC(sc.get(&mask, (size_t)nt));
   SDTFE_RC(sc.get(&cnt, (size_t)nt));
   SDTFE_RC(sc.get(&off, (size_t)nt));
   SDTFE_RC(sc.get(&tmp, scan_tmp_ints(nt)));
-  if (nt > 0)
-    hull_mask_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
-        reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, mask, cnt);
+  SDTFE_RC(sc.get(&part, 4 * (size_t)nblk));
+  SDTFE_RC(sc.get(&box, 5));
+  // the forward hull's triangle count and bounding box, read back together
   int n_tri = 0;
-  SDTFE_RC(scan_exclusive(cnt, off, nt, tmp, &n_tri, s));
+  double bb[4] = {0, 0, 0, 0};
+  if (nt > 0) {
+    hull_mask_kernel<<<nblk, 256, 0, s>>>(reinterpret_cast<const double4 *>(t->pos4), t->tets,
+                                          nt, mask, cnt, part);
+    SDTFE_RC(scan_exclusive(cnt, off, nt, tmp, nullptr, s));
+    hull_box_kernel<<<1, 256, 0, s>>>(part, (int)nblk, scan_total_device(tmp, nt), box);
+    SDTFE_HIP(hipGetLastError());
+    double h[5];
+    SDTFE_HIP(hipMemcpyAsync(h, box, sizeof h, hipMemcpyDeviceToHost, s));
+    SDTFE_HIP(hipStreamSynchronize(s));
+    n_tri = (int)h[4];
+    if (n_tri > 0)
+      for (int j = 0; j < 4; ++j) bb[j] = h[j];
+  }
   t->n_tri = n_tri;
   SDTFE_RC(dev_alloc(t, &t->tri_xy, 6 * (size_t)n_tri));
   SDTFE_RC(dev_alloc(t, &t->tri_tet, (size_t)n_tri));
@@ -554,13 +606,7 @@ int build_entry_grid(sdtfe_tess *t) {
   g.tri_rid = t->tri_rid;
   g.n_tri = n_tri;
   g.nx = g.ny = 1;
-  double bb[4] = {0, 0, 0, 0};
   if (n_tri > 0) {
-    double *bbd = nullptr;
-    SDTFE_RC(sc.get(&bbd, 4));
-    tri_bbox_kernel<<<1, 256, 0, s>>>(t->tri_xy, n_tri, bbd);
-    SDTFE_HIP(hipMemcpyAsync(bb, bbd, sizeof bb, hipMemcpyDeviceToHost, s));
-    SDTFE_HIP(hipStreamSynchronize(s));
     const double w = bb[2] - bb[0], h = bb[3] - bb[1];
     // about two cells per triangle, square cells, at most 4096 per side
     const double area = w * h;
@@ -651,7 +697,7 @@ int hull_mask_host(const double *pd, int n, const int *tetra, int nt, unsigned c
   SDTFE_RC(sc.get(&dmask, (size_t)nt));
   SDTFE_RC(sc.get(&cnt, (size_t)nt));
   hull_mask_kernel<<<blocks_for(nt, 256), 256, 0, h.s>>>(reinterpret_cast<const double4 *>(h.pos),
-                                                         h.tets, nt, dmask, cnt);
+                                                         h.tets, nt, dmask, cnt, nullptr);
   SDTFE_HIP(hipGetLastError());
   SDTFE_HIP(hipMemcpyAsync(mask, dmask, (size_t)nt, hipMemcpyDeviceToHost, h.s));
   SDTFE_HIP(hipStreamSynchronize(h.s));

@@ -28,6 +28,9 @@ int hip_check(hipError_t e, const char *what);
 size_t scan_tmp_ints(int64_t n);
 int scan_exclusive(const int *in, int *out, int64_t n, int *tmp, int *total_host,
                    hipStream_t s);
+// The total of a scan of n > 0 items in device memory (inside tmp), for a
+// caller that reads it back with other values.
+const int *scan_total_device(const int *tmp, int64_t n);
 
 struct DevBuf {
   void *p = nullptr;

@@ -53,6 +53,7 @@ struct Request {
   double *out = nullptr;  // caller's host map, accumulated into
   int64_t footprint = 0;
   sdtfe_tess *tess = nullptr;
+  hipStream_t stream = nullptr;  // of the server's pool: set-up, map buffer
   double *d_map = nullptr;
   int n_tiles = 0, next_tile = 0, lpp = 1;
   Clock::time_point t_submit, t_admit, t_start, t_done;
@@ -165,7 +166,7 @@ int copy_out(sdtfe_server *sv, Request &r, Pinned &pb) {
 // are what the waiter sees.  The caller holds no lock, and the request's
 // device work is complete (its frees are ordered on its stream).
 void finish(sdtfe_server *sv, const std::shared_ptr<Request> &r, int status, std::string error) {
-  if (r->d_map) dev_free(r->d_map, r->tess->stream);
+  if (r->d_map) dev_free(r->d_map, r->stream);
   r->d_map = nullptr;
   sdtfe_tess_destroy(r->tess);
   r->tess = nullptr;
@@ -214,21 +215,25 @@ void setup_loop(sdtfe_server *sv, int thread_no) {
       r->t_admit = Clock::now();
       stream = sv->streams[(2 * (size_t)thread_no + (turn++ & 1)) % n_streams];
     }
+    // the map buffer first, cleared on the same stream: the tessellation's
+    // set-up ends by synchronising that stream, which covers the clearing
+    r->stream = stream;
+    const size_t npix = (size_t)r->params.grid_dim * r->params.grid_dim;
+    int rc = 0;
+    if (dev_malloc((void **)&r->d_map, sizeof(double) * npix, stream) != hipSuccess) {
+      r->d_map = nullptr;
+      rc = fail(SDTFE_ENOMEM, "map buffer");
+    } else if (hipMemsetAsync(r->d_map, 0, sizeof(double) * npix, stream) != hipSuccess) {
+      rc = fail(SDTFE_EHIP, "clearing the map buffer failed");
+    }
     // a mixed-precision server never walks the FP64 face records
-    int rc = tess_create_staged(r->pd.data(), (int)(r->pd.size() / 4), r->tets.data(),
-                                (int)(r->tets.size() / 8), r->p_mass, &st, stream,
-                                sv->cfg.precision == SDTFE_PREC_FP64, &r->tess);
+    if (rc == 0)
+      rc = tess_create_staged(r->pd.data(), (int)(r->pd.size() / 4), r->tets.data(),
+                              (int)(r->tets.size() / 8), r->p_mass, &st, stream,
+                              sv->cfg.precision == SDTFE_PREC_FP64, &r->tess);
     // the host copies are uploaded: release them now
     std::vector<double>().swap(r->pd);
     std::vector<int>().swap(r->tets);
-    const size_t npix = (size_t)r->params.grid_dim * r->params.grid_dim;
-    if (rc == 0 &&
-        dev_malloc((void **)&r->d_map, sizeof(double) * npix, r->tess->stream) != hipSuccess)
-      rc = fail(SDTFE_ENOMEM, "map buffer");
-    if (rc == 0 && (hipMemsetAsync(r->d_map, 0, sizeof(double) * npix, r->tess->stream) !=
-                        hipSuccess ||
-                    hipStreamSynchronize(r->tess->stream) != hipSuccess))
-      rc = fail(SDTFE_EHIP, "clearing the map buffer failed");
     if (rc == 0) {
       r->lpp = sv->cfg.lanes_per_pixel > 0 ? sv->cfg.lanes_per_pixel
                                            : map_lanes(r->params.n_mc_samp);
