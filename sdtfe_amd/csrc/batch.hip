@@ -52,6 +52,7 @@ struct BatchItem {
 struct BatchHdr {
   int n_items;
   int total_tiles;
+  int active_blocks;    // blocks of the persistent kernel that take tiles
   int share[kXcd + 1];  // tile-space cut points
   DeferQueue dq;        // mixed walk: pixels for the FP64 pass (count: counters[kXcd])
 };
@@ -116,6 +117,9 @@ __device__ __forceinline__ void batch_body(const BatchItem *__restrict__ items,
                                            const BatchHdr *__restrict__ hdr, int *counters,
                                            unsigned long long *item_stats, double *lds,
                                            int &s_tile, uint8_t *faces) {
+  // blocks beyond active_blocks leave at once, freeing their slots for
+  // other streams' kernels (the same for every thread of a block)
+  if ((int)blockIdx.x >= hdr->active_blocks) return;
   if (kMixed && threadIdx.x < kFaceCodes) faces[threadIdx.x] = (uint8_t)face_code((int)threadIdx.x);
   const int home = blockIdx.x % kXcd;
   const int n_items = hdr->n_items;
@@ -274,8 +278,9 @@ struct sdtfe_batch {
   sdtfe::BatchHdr *d_hdr = nullptr;
   int *d_counters = nullptr;
   unsigned long long *d_stats = nullptr;
-  sdtfe::BatchItem *h_items = nullptr;  // pinned
-  sdtfe::BatchHdr *h_hdr = nullptr;     // pinned
+  sdtfe::BatchItem *h_items = nullptr;    // pinned
+  sdtfe::BatchHdr *h_hdr = nullptr;       // pinned
+  unsigned long long *h_stats = nullptr;  // pinned, the items' counters read back
   int cap = 0;
   // captured launch
   hipGraph_t graph = nullptr;
@@ -284,6 +289,7 @@ struct sdtfe_batch {
   int graph_blocks = 0;
   size_t graph_lds = 0;
   int blocks_fp64 = 0, blocks_mixed = 0, blocks_wide = 0;
+  int cus = 0;  // compute units of the device
   sdtfe::Deferred *d_defer = nullptr;  // mixed walk: pixels for the FP64 pass
   size_t defer_cap = 0;
 };
@@ -304,23 +310,26 @@ int ensure_cap(sdtfe_batch *b, int n) {
   if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_stats) (void)hipFree(b->d_stats);
   if (b->h_items) (void)hipHostFree(b->h_items);
+  if (b->h_stats) (void)hipHostFree(b->h_stats);
   b->d_items = nullptr;
   b->d_stats = nullptr;
   b->h_items = nullptr;
+  b->h_stats = nullptr;
   b->cap = 0;
   const int cap = std::max(n, 64);
+  const size_t stat_bytes = sizeof(unsigned long long) * ST_COUNT * cap;
   if (hipMalloc((void **)&b->d_items, sizeof(BatchItem) * cap) != hipSuccess ||
-      hipMalloc((void **)&b->d_stats, sizeof(unsigned long long) * ST_COUNT * cap) !=
-          hipSuccess ||
+      hipMalloc((void **)&b->d_stats, stat_bytes) != hipSuccess ||
       hipHostMalloc((void **)&b->h_items, sizeof(BatchItem) * cap, hipHostMallocDefault) !=
-          hipSuccess)
+          hipSuccess ||
+      hipHostMalloc((void **)&b->h_stats, stat_bytes, hipHostMallocDefault) != hipSuccess)
     return fail(SDTFE_ENOMEM, "batch buffers");
   b->cap = cap;
   return 0;
 }
 
 template <bool kMixed, bool kWide>
-int blocks_for(sdtfe_batch *, size_t lds) {
+int blocks_for(sdtfe_batch *b, size_t lds) {
   int per_cu = 0;
   SDTFE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel<kMixed, kWide>(),
                                                          256, lds));
@@ -328,6 +337,7 @@ int blocks_for(sdtfe_batch *, size_t lds) {
   SDTFE_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   SDTFE_HIP(hipGetDeviceProperties(&prop, dev));
+  b->cus = prop.multiProcessorCount;
   return std::max(1, per_cu) * prop.multiProcessorCount;
 }
 
@@ -375,7 +385,7 @@ int batch_tiles(const sdtfe_map_params &p, int lanes_per_pixel) {
 int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
                      const sdtfe_map_params *params, double *const *rho_device,
                      const int *lanes_per_pixel, const int *tile_begin, const int *tile_count,
-                     sdtfe_stats *item_stats, sdtfe_stats *total) {
+                     sdtfe_stats *item_stats, sdtfe_stats *total, bool leave_room) {
   if (!b || n < 0 || (n && (!tess || !params || !rho_device)))
     return fail(SDTFE_EINVAL, "bad argument");
   if (n == 0) return 0;
@@ -466,6 +476,7 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
                     : (wide ? blocks_for<true, true>(b, kLdsBytes)
                             : blocks_for<true, false>(b, kLdsBytes));
   if (blocks <= 0) return fail(SDTFE_EHIP, "occupancy query failed");
+  h.active_blocks = leave_room && blocks > b->cus ? blocks - b->cus : blocks;
   // one LDS size for every launch, so one captured graph serves every batch
   if (!b->exec || b->graph_mixed != mixed || b->graph_wide != wide ||
       b->graph_blocks != blocks) {
@@ -481,20 +492,18 @@ int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
   SDTFE_HIP(hipEventRecord(b->ev0, b->stream));
   SDTFE_HIP(hipGraphLaunch(b->exec, b->stream));
   SDTFE_HIP(hipEventRecord(b->ev1, b->stream));
-  SDTFE_HIP(hipEventSynchronize(b->ev1));
-  float ms = 0;
-  SDTFE_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  std::vector<unsigned long long> st((size_t)n * ST_COUNT);
-  // on the batch's stream: the null stream would order the copy against
-  // other work of the process (a server's set-up threads issue theirs
-  // meanwhile)
-  SDTFE_HIP(hipMemcpyAsync(st.data(), b->d_stats, sizeof(unsigned long long) * st.size(),
+  // the items' counters follow the launch on its stream into pinned memory,
+  // so one wait covers both (the null stream would order the copy against
+  // other work of the process, such as a server's set-up threads)
+  SDTFE_HIP(hipMemcpyAsync(b->h_stats, b->d_stats, sizeof(unsigned long long) * ST_COUNT * n,
                            hipMemcpyDeviceToHost, b->stream));
   SDTFE_HIP(hipStreamSynchronize(b->stream));
+  float ms = 0;
+  SDTFE_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   sdtfe_stats sum{};
   for (int i = 0; i < n; ++i) {
     sdtfe_stats s{};
-    fill_stats_pub(st.data() + (size_t)i * ST_COUNT, &s);
+    fill_stats_pub(b->h_stats + (size_t)i * ST_COUNT, &s);
     if (item_stats) item_stats[i] = s;
     sum.samples += s.samples;
     sum.missed += s.missed;
@@ -555,6 +564,7 @@ void sdtfe_batch_destroy(sdtfe_batch *b) {
   if (b->d_defer) (void)hipFree(b->d_defer);
   if (b->h_items) (void)hipHostFree(b->h_items);
   if (b->h_hdr) (void)hipHostFree(b->h_hdr);
+  if (b->h_stats) (void)hipHostFree(b->h_stats);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -317,9 +317,13 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
   *out = nullptr;
   if (n < 0 || nt < 0) return fail(SDTFE_EINVAL, "negative size");
   if ((n > 0 && !pd) || (nt > 0 && !tetra)) return fail(SDTFE_EINVAL, "null input array");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-    return fail(SDTFE_ENODEV, "no HIP device visible");
+  // the device count once per process: the query costs a round trip to the
+  // driver, and a server creates a tessellation per request
+  static const int ndev = [] {
+    int k = 0;
+    return hipGetDeviceCount(&k) == hipSuccess ? k : 0;
+  }();
+  if (ndev == 0) return fail(SDTFE_ENODEV, "no HIP device visible");
   auto t0 = Clock::now();
   sdtfe_tess *t = new sdtfe_tess();
   t->n = n;

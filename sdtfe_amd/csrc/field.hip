@@ -189,25 +189,45 @@ __global__ void fill_incidence_kernel(const int *tets, int nt, const int *off, i
   inc[off[v.w] + atomicAdd(cursor + v.w, 1)] = t;
 }
 
-// One vertex per thread: put its incident tets in ascending order (the
-// order the reference adds them) and sum their volumes.
-__global__ void vertex_density_kernel(double4 *pos, int n, const int *off, int *inc,
-                                      const double *vol, float m4) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  const int b = off[v], e = off[v + 1];
-  for (int i = b + 1; i < e; ++i) {
-    int key = inc[i];
-    int j = i - 1;
-    while (j >= b && inc[j] > key) {
-      inc[j + 1] = inc[j];
-      --j;
-    }
-    inc[j + 1] = key;
-  }
+// One wave per vertex: the volumes of its incident tets added in ascending
+// tet order, the order the reference adds them (it loops over the tets).
+// Up to 64 incident tets: lane k holds the k-th, finds its rank among them
+// across the wave (ties, from a tet naming a vertex twice, by lane), and the
+// volumes are added rank by rank.  Beyond 64 (rare), one lane puts them in
+// order in place and adds them.
+__global__ __launch_bounds__(256) void vertex_density_kernel(double4 *pos, int n, const int *off,
+                                                             int *inc, const double *vol,
+                                                             float m4) {
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (v >= n) return;  // the same for the whole wave
+  const int b = off[v], e = off[v + 1], d = e - b;
   double w = 0;
-  for (int i = b; i < e; ++i) w += vol[inc[i]];
-  pos[v].w = (double)m4 / w;
+  if (d <= 64) {
+    const int key = lane < d ? inc[b + lane] : INT_MAX;
+    int rank = 0;
+    for (int k = 0; k < d; ++k) {
+      const int o = __shfl(key, k, 64);
+      rank += o < key || (o == key && k < lane);
+    }
+    const double vv = lane < d ? vol[key] : 0.0;
+    for (int r = 0; r < d; ++r) {
+      const unsigned long long m = __ballot(lane < d && rank == r);
+      w += __shfl(vv, __ffsll(m) - 1, 64);
+    }
+  } else if (lane == 0) {
+    for (int i = b + 1; i < e; ++i) {
+      const int key = inc[i];
+      int j = i - 1;
+      while (j >= b && inc[j] > key) {
+        inc[j + 1] = inc[j];
+        --j;
+      }
+      inc[j + 1] = key;
+    }
+    for (int i = b; i < e; ++i) w += vol[inc[i]];
+  }
+  if (lane == 0) pos[v].w = (double)m4 / w;
 }
 
 __device__ inline double det3(double a, double b, double c, double d, double e, double f,
@@ -521,7 +541,7 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra, Staging *st) 
     fill_incidence_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(t->tets, nt, off, cursor, inc);
   const float m4 = t->p_mass * 4;  // a float product, src/triangulate.c:224
   if (n > 0 && !t->given_field)
-    vertex_density_kernel<<<blocks_for(n, 256), 256, 0, s>>>(
+    vertex_density_kernel<<<blocks_for(64 * (int64_t)n, 256), 256, 0, s>>>(
         reinterpret_cast<double4 *>(t->pos4), n, off, inc, t->vol, m4);
 
   // centre and half-extent of the particles' bounding box

@@ -94,12 +94,14 @@ MapGeom make_geom_pub(const sdtfe_tess *t, const sdtfe_map_params &p);
 void fill_stats_pub(const unsigned long long *c, sdtfe_stats *s);
 // batch.hip: tiles of one map in a batch, and a batched launch over tile
 // ranges [tile_begin[i], tile_begin[i] + tile_count[i]) of each item (null
-// ranges: whole maps).
+// ranges: whole maps).  leave_room: the persistent kernel leaves one block
+// slot per CU unused, so kernels of other streams (a server's set-ups) run
+// beside it instead of after it.
 int batch_tiles(const sdtfe_map_params &p, int lanes_per_pixel);
 int batch_run_ranges(sdtfe_batch *b, int n, sdtfe_tess *const *tess,
                      const sdtfe_map_params *params, double *const *rho_device,
                      const int *lanes_per_pixel, const int *tile_begin, const int *tile_count,
-                     sdtfe_stats *item_stats, sdtfe_stats *total);
+                     sdtfe_stats *item_stats, sdtfe_stats *total, bool leave_room = false);
 // Sizes a batch's item buffers for n items up front, so later launches of up
 // to n items neither reallocate them nor capture the graph again.
 int batch_reserve(sdtfe_batch *b, int n);

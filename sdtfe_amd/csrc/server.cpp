@@ -15,7 +15,9 @@
 //   walk thread   repeatedly launches one persistent batch over the next
 //                 tiles of every resident request (at most tile_budget tiles
 //                 per launch, shared round-robin), so a request admitted
-//                 while others are on the GPU joins the next launch; a
+//                 while others are on the GPU joins the next launch; while
+//                 requests are being set up, a launch leaves one block slot
+//                 per CU to their kernels, which then do not wait for it; a
 //                 request whose tiles are all done goes to the completion
 //                 thread, and the next launch does not wait for it.
 //   completion    copies a finished map out on a stream of its own (through
@@ -260,11 +262,13 @@ void walk_loop(sdtfe_server *sv) {
   size_t rr = 0;  // round-robin start
   for (;;) {
     std::vector<std::shared_ptr<Request>> items;
+    bool setups = false;  // set-up work may come while this launch runs
     {
       std::unique_lock<std::mutex> lk(sv->mu);
       sv->cv_walk.wait(lk, [&] { return sv->stop || !sv->active.empty(); });
       if (sv->stop) return;
       items = sv->active;
+      setups = sv->n_setup > 0 || !sv->queued.empty();
     }
     // share the tile budget: every request gets an equal slice, leftovers
     // go round-robin
@@ -316,7 +320,7 @@ void walk_loop(sdtfe_server *sv) {
     const int rc = ts.empty() ? 0
                               : batch_run_ranges(sv->batch, (int)ts.size(), ts.data(), ps.data(),
                                                  outs.data(), lp.data(), b0.data(), cnt.data(),
-                                                 st.data(), &tot);
+                                                 st.data(), &tot, setups);
     std::vector<std::shared_ptr<Request>> finished, empty;  // empty: had no tiles
     {
       std::lock_guard<std::mutex> g(sv->mu);
