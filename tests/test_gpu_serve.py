@@ -97,3 +97,22 @@ def test_failed_request_reports_its_error(requests):
             sv.wait(ids[1])
         _, i2 = sv.wait(ids[2])
     assert i0["status"] == 0 and i2["status"] == 0
+
+
+def test_close_drains_unwaited_requests(requests):
+    """Closing a server with requests nobody waited for drains it: every map
+    is walked, copied out into the caller's array and freed before close
+    returns, and equals the same map computed alone (FP64: bit for bit)."""
+    _say("[gpu] server: close with requests never waited for")
+    reqs = requests[:6]
+    alone = []
+    for r in reqs:
+        with capi.Tessellation(r.pd, r.tetra) as t:
+            alone.append(t.map(r.grid, r.samples, seed=r.seed)[0])
+    outs = [np.zeros(r.grid * r.grid) for r in reqs]
+    sv = capi.Server(max_active=4, tile_budget=64)
+    for r, o in zip(reqs, outs):
+        sv.submit(r.pd, r.tetra, serve.params_for(sv, r), rho=o)
+    sv.close()
+    for got, ref in zip(outs, alone):
+        np.testing.assert_array_equal(got, ref)
