@@ -247,6 +247,67 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st)
   return 0;
 }
 
+int download(const void *src, size_t bytes, hipStream_t s, Staging *st,
+             const std::function<void(const char *, size_t, size_t)> &consume) {
+  if (!bytes) return 0;
+  if (!st || !st->bytes || !st->buf[0] || !st->buf[1]) {
+    std::vector<char> tmp(bytes);
+    SDTFE_HIP(hipMemcpyAsync(tmp.data(), src, bytes, hipMemcpyDeviceToHost, s));
+    SDTFE_HIP(hipStreamSynchronize(s));
+    consume(tmp.data(), 0, bytes);
+    return 0;
+  }
+  const size_t slice = st->bytes, n = (bytes + slice - 1) / slice;
+  auto issue = [&](size_t k) -> int {
+    const size_t off = k * slice;
+    SDTFE_HIP(hipEventSynchronize(st->ev[k & 1]));  // the buffer's previous copy is done
+    SDTFE_HIP(hipMemcpyAsync(st->buf[k & 1], static_cast<const char *>(src) + off,
+                             std::min(slice, bytes - off), hipMemcpyDeviceToHost, s));
+    SDTFE_HIP(hipEventRecord(st->ev[k & 1], s));
+    return 0;
+  };
+  if (int rc = issue(0)) return rc;
+  for (size_t k = 0; k < n; ++k) {
+    // the next slice goes into the buffer the host finished with last time
+    if (k + 1 < n)
+      if (int rc = issue(k + 1)) return rc;
+    SDTFE_HIP(hipEventSynchronize(st->ev[k & 1]));
+    const size_t off = k * slice;
+    consume(static_cast<const char *>(st->buf[k & 1]), off, std::min(slice, bytes - off));
+  }
+  return 0;
+}
+
+namespace {
+
+constexpr size_t kPoolSlice = (size_t)8 << 20;
+std::mutex g_staging_mu;
+std::vector<Staging *> g_staging_idle;
+
+}  // namespace
+
+PooledStaging::PooledStaging() : st(nullptr) {
+  {
+    std::lock_guard<std::mutex> g(g_staging_mu);
+    if (!g_staging_idle.empty()) {
+      st = g_staging_idle.back();
+      g_staging_idle.pop_back();
+      return;
+    }
+  }
+  st = new Staging();
+  const std::string keep = g_err;  // a failed pinned allocation is not the call's error
+  if (staging_init(*st, kPoolSlice)) staging_free(*st);  // pageable copies then
+  g_err = keep;
+}
+
+PooledStaging::~PooledStaging() {
+  for (int k = 0; k < 2; ++k)
+    if (st->ev[k]) (void)hipEventSynchronize(st->ev[k]);  // back to the pool idle
+  std::lock_guard<std::mutex> g(g_staging_mu);
+  g_staging_idle.push_back(st);
+}
+
 int tess_create_staged(const double *particle_data, int n, const int *tetra_data, int nt,
                        float p_mass, Staging *st, hipStream_t stream, bool fp64_faces,
                        sdtfe_tess **out) {
@@ -345,7 +406,14 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
   if (!rc && dev_malloc((void **)&t->stats, sizeof(unsigned long long) * ST_COUNT, t->stream) !=
                  hipSuccess)
     rc = fail(SDTFE_ENOMEM, "stats allocation failed");
-  if (!rc) rc = build_field(t, pd, tetra, st);
+  if (!rc) {
+    if (st) {
+      rc = build_field(t, pd, tetra, st);
+    } else {
+      PooledStaging ps;
+      rc = build_field(t, pd, tetra, ps.st);
+    }
+  }
   if (!rc) rc = build_entry_grid(t);
   if (rc) {
     free_tess(t);
@@ -385,6 +453,16 @@ int run_map(sdtfe_tess *t, const sdtfe_map_params *p, double *out, int *navg, in
 
 using namespace sdtfe;
 
+namespace {
+
+// Column 3 of the particle rows (x, y, z, rho) into a contiguous array.
+__global__ void density_column_kernel(const double *pos4, int n, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = pos4[4 * (size_t)i + 3];
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *sdtfe_last_error(void) { return g_err.c_str(); }
@@ -421,21 +499,32 @@ int sdtfe_tess_get_info(const sdtfe_tess *t, sdtfe_tess_info *info) {
 int sdtfe_tess_vertex_density(const sdtfe_tess *t, double *rho) {
   if (!t || (!rho && t->n)) return fail(SDTFE_EINVAL, "null argument");
   if (t->n == 0) return 0;
-  std::vector<double> tmp(4 * (size_t)t->n);
-  SDTFE_HIP(hipMemcpyAsync(tmp.data(), t->pos4, sizeof(double) * tmp.size(),
-                           hipMemcpyDeviceToHost, t->stream));
-  SDTFE_HIP(hipStreamSynchronize(t->stream));
-  for (int i = 0; i < t->n; ++i) rho[i] = tmp[4 * (size_t)i + 3];
-  return 0;
+  // the density column alone goes back (a quarter of the particle rows)
+  double *col = nullptr;
+  if (dev_malloc((void **)&col, sizeof(double) * (size_t)t->n, t->stream) != hipSuccess)
+    return fail(SDTFE_ENOMEM, "density read-back buffer allocation failed");
+  density_column_kernel<<<(unsigned)((t->n + 255) / 256), 256, 0, t->stream>>>(t->pos4, t->n,
+                                                                              col);
+  int rc = hip_check(hipGetLastError(), "density_column_kernel");
+  if (!rc) {
+    PooledStaging ps;
+    rc = download(col, sizeof(double) * (size_t)t->n, t->stream, ps.st,
+                  [rho](const char *h, size_t off, size_t n) {
+                    std::memcpy(reinterpret_cast<char *>(rho) + off, h, n);
+                  });
+  }
+  dev_free(col, t->stream);
+  return rc;
 }
 
 int sdtfe_tess_gradients(const sdtfe_tess *t, double *nabla) {
   if (!t || (!nabla && t->nt)) return fail(SDTFE_EINVAL, "null argument");
   if (t->nt == 0) return 0;
-  SDTFE_HIP(hipMemcpyAsync(nabla, t->nabla, sizeof(double) * 3 * (size_t)t->nt,
-                           hipMemcpyDeviceToHost, t->stream));
-  SDTFE_HIP(hipStreamSynchronize(t->stream));
-  return 0;
+  PooledStaging ps;
+  return download(t->nabla, sizeof(double) * 3 * (size_t)t->nt, t->stream, ps.st,
+                  [nabla](const char *h, size_t off, size_t n) {
+                    std::memcpy(reinterpret_cast<char *>(nabla) + off, h, n);
+                  });
 }
 
 int sdtfe_tess_triangles(const sdtfe_tess *t, int *rows, int capacity) {
@@ -546,14 +635,21 @@ int sdtfe_map(sdtfe_tess *t, const sdtfe_map_params *p, double *rho, int *n_avg,
   if ((rc = ensure_map_buf(t, npix))) return rc;
   double t_walk = 0, t_kernel = 0;
   if ((rc = run_map(t, p, t->map_buf, t->navg_buf, 0, &t_walk, &t_kernel))) return rc;
-  std::vector<double> val(npix);
-  SDTFE_HIP(hipMemcpyAsync(val.data(), t->map_buf, sizeof(double) * npix, hipMemcpyDeviceToHost,
-                           t->stream));
-  if (n_avg)
-    SDTFE_HIP(hipMemcpyAsync(n_avg, t->navg_buf, sizeof(int) * npix, hipMemcpyDeviceToHost,
-                             t->stream));
-  SDTFE_HIP(hipStreamSynchronize(t->stream));
-  for (size_t i = 0; i < npix; ++i) rho[i] += val[i];  // src/dtfe.c:642-645
+  {
+    PooledStaging ps;
+    rc = download(t->map_buf, sizeof(double) * npix, t->stream, ps.st,
+                  [rho](const char *h, size_t off, size_t n) {
+                    const double *v = reinterpret_cast<const double *>(h);
+                    double *r = rho + off / sizeof(double);
+                    for (size_t i = 0; i < n / sizeof(double); ++i) r[i] += v[i];  // src/dtfe.c:642-645
+                  });
+    if (!rc && n_avg)
+      rc = download(t->navg_buf, sizeof(int) * npix, t->stream, ps.st,
+                    [n_avg](const char *h, size_t off, size_t n) {
+                      std::memcpy(reinterpret_cast<char *>(n_avg) + off, h, n);
+                    });
+    if (rc) return rc;
+  }
   if ((rc = read_stats(t, stats))) return rc;
   if (stats) {
     stats->t_walk_ms = t_walk;
@@ -626,14 +722,16 @@ int sdtfe_volume(sdtfe_tess *t, const sdtfe_volume_params *p, double *grid,
   // the walk adds into the caller's values in the reference's order, so
   // they go up first
   double t_walk = 0;
-  rc = hip_check(hipMemcpyAsync(d, grid, sizeof(double) * cells, hipMemcpyHostToDevice,
-                                t->stream),
-                 "grid upload");
-  if (!rc) rc = run_volume(t, p, d, &t_walk);
-  if (!rc)
-    rc = hip_check(hipMemcpyAsync(grid, d, sizeof(double) * cells, hipMemcpyDeviceToHost,
-                                  t->stream),
-                   "grid download");
+  {
+    PooledStaging ps;
+    rc = upload(d, grid, sizeof(double) * cells, t->stream, ps.st);
+    if (!rc) rc = run_volume(t, p, d, &t_walk);
+    if (!rc)
+      rc = download(d, sizeof(double) * cells, t->stream, ps.st,
+                    [grid](const char *h, size_t off, size_t n) {
+                      std::memcpy(reinterpret_cast<char *>(grid) + off, h, n);
+                    });
+  }
   dev_free(d, t->stream);
   if (!rc) rc = hip_check(hipStreamSynchronize(t->stream), "sync");
   if (rc) return rc;

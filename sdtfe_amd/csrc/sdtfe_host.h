@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 
 #include "../../include/sdtfe_amd.h"
@@ -122,6 +123,22 @@ struct Staging {
 int staging_init(Staging &st, size_t bytes);
 void staging_free(Staging &st);
 int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st);
+// Device-to-host read-back through st's pinned buffers: the copy of one
+// slice is in flight while consume(host, offset, n) takes the one before
+// (offset and n in bytes).  Without pinned buffers, one pageable copy.
+int download(const void *src, size_t bytes, hipStream_t s, Staging *st,
+             const std::function<void(const char *, size_t, size_t)> &consume);
+// Pinned staging for calls that bring none (the drop-ins, sdtfe_tess_create,
+// maps and grids read back to the host): taken from a process-wide pool for
+// the call and returned idle; the pool's buffers are kept until the process
+// ends, one pair per call running at once.
+struct PooledStaging {
+  Staging *st;
+  PooledStaging();
+  ~PooledStaging();
+  PooledStaging(const PooledStaging &) = delete;
+  PooledStaging &operator=(const PooledStaging &) = delete;
+};
 // A tessellation as sdtfe_tess_create makes it, uploading through st, on
 // stream (the caller's, kept when the tessellation is destroyed) or, when
 // null, on a stream of its own; without the FP64 walk's face records when
