@@ -7,8 +7,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sdtfe_host.h"
@@ -231,8 +233,55 @@ void staging_free(Staging &st) {
   st.bytes = 0;  // uploads then go straight from pageable memory
 }
 
+namespace {
+
+// Uploads of at least this many bytes are split into kUploadParts ranges,
+// each copied through its own pinned buffers by its own host thread (the
+// host copy into pinned memory, not the transfer, limits one thread).
+constexpr size_t kParallelUpload = (size_t)64 << 20;
+constexpr int kUploadParts = 4;
+
+int upload_serial(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st);
+
+int upload_parallel(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st) {
+  std::vector<std::unique_ptr<PooledStaging>> more;
+  for (int k = 1; k < kUploadParts; ++k) more.push_back(std::make_unique<PooledStaging>());
+  int rc[kUploadParts] = {};
+  std::string err[kUploadParts];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  auto part = [&](int k) {
+    if (k > 0) (void)hipSetDevice(dev);  // a new thread starts on device 0
+    const size_t lo = bytes * k / kUploadParts, hi = bytes * (k + 1) / kUploadParts;
+    Staging *sk = k == 0 ? st : more[k - 1]->st;
+    rc[k] = upload_serial(static_cast<char *>(dst) + lo, static_cast<const char *>(src) + lo,
+                          hi - lo, s, sk);
+    if (rc[k]) err[k] = g_err;  // the worker's message, for the caller's thread
+  };
+  std::vector<std::thread> th;
+  for (int k = 1; k < kUploadParts; ++k) th.emplace_back(part, k);
+  part(0);
+  for (std::thread &x : th) x.join();
+  for (int k = 0; k < kUploadParts; ++k)
+    if (rc[k]) return fail(rc[k], err[k]);
+  return 0;
+}
+
+}  // namespace
+
 int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st) {
   if (!st || !st->bytes || !st->buf[0] || !st->buf[1]) {
+    SDTFE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return 0;
+  }
+  if (bytes >= kParallelUpload) return upload_parallel(dst, src, bytes, s, st);
+  return upload_serial(dst, src, bytes, s, st);
+}
+
+namespace {
+
+int upload_serial(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st) {
+  if (!st->bytes || !st->buf[0] || !st->buf[1]) {
     SDTFE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
     return 0;
   }
@@ -246,6 +295,8 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st)
   }
   return 0;
 }
+
+}  // namespace
 
 int download(const void *src, size_t bytes, hipStream_t s, Staging *st,
              const std::function<void(const char *, size_t, size_t)> &consume) {
@@ -282,18 +333,22 @@ namespace {
 
 constexpr size_t kPoolSlice = (size_t)8 << 20;
 std::mutex g_staging_mu;
-std::vector<Staging *> g_staging_idle;
+std::vector<std::pair<int, Staging *>> g_staging_idle;  // (device, buffers)
 
 }  // namespace
 
-PooledStaging::PooledStaging() : st(nullptr) {
+// Buffers and events of the device current when taken (events belong to a
+// device).
+PooledStaging::PooledStaging() : st(nullptr), dev(0) {
+  (void)hipGetDevice(&dev);
   {
     std::lock_guard<std::mutex> g(g_staging_mu);
-    if (!g_staging_idle.empty()) {
-      st = g_staging_idle.back();
-      g_staging_idle.pop_back();
-      return;
-    }
+    for (size_t i = g_staging_idle.size(); i-- > 0;)
+      if (g_staging_idle[i].first == dev) {
+        st = g_staging_idle[i].second;
+        g_staging_idle.erase(g_staging_idle.begin() + (long)i);
+        return;
+      }
   }
   st = new Staging();
   const std::string keep = g_err;  // a failed pinned allocation is not the call's error
@@ -305,7 +360,7 @@ PooledStaging::~PooledStaging() {
   for (int k = 0; k < 2; ++k)
     if (st->ev[k]) (void)hipEventSynchronize(st->ev[k]);  // back to the pool idle
   std::lock_guard<std::mutex> g(g_staging_mu);
-  g_staging_idle.push_back(st);
+  g_staging_idle.emplace_back(dev, st);
 }
 
 int tess_create_staged(const double *particle_data, int n, const int *tetra_data, int nt,

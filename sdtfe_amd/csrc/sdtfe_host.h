@@ -129,11 +129,13 @@ int upload(void *dst, const void *src, size_t bytes, hipStream_t s, Staging *st)
 int download(const void *src, size_t bytes, hipStream_t s, Staging *st,
              const std::function<void(const char *, size_t, size_t)> &consume);
 // Pinned staging for calls that bring none (the drop-ins, sdtfe_tess_create,
-// maps and grids read back to the host): taken from a process-wide pool for
-// the call and returned idle; the pool's buffers are kept until the process
-// ends, one pair per call running at once.
+// maps and grids read back to the host): taken from a process-wide pool (per
+// device) for the call and returned idle; the pool's buffers are kept until
+// the process ends, one pair per call (four per large upload) running at
+// once.  Uploads of 64 MB or more are split over four host threads.
 struct PooledStaging {
   Staging *st;
+  int dev;  // the device its events belong to
   PooledStaging();
   ~PooledStaging();
   PooledStaging(const PooledStaging &) = delete;
