@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -259,7 +260,13 @@ int upload_parallel(void *dst, const void *src, size_t bytes, hipStream_t s, Sta
     if (rc[k]) err[k] = g_err;  // the worker's message, for the caller's thread
   };
   std::vector<std::thread> th;
-  for (int k = 1; k < kUploadParts; ++k) th.emplace_back(part, k);
+  for (int k = 1; k < kUploadParts; ++k) {
+    try {
+      th.emplace_back(part, k);
+    } catch (const std::exception &) {
+      part(k);  // no thread to be had: this thread copies the part
+    }
+  }
   part(0);
   for (std::thread &x : th) x.join();
   for (int k = 0; k < kUploadParts; ++k)
