@@ -39,7 +39,8 @@ void launch_map_fp64(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, in
 void launch_trace_fp64(sdtfe_tess *t, const MapGeom &g, int n, const double *x, const double *y,
                        const int *entry, const uint64_t *pixel, const uint32_t *sample,
                        double *value, int *crossed);
-int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate);
+int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate,
+                     int *h_count);
 void launch_trace_mixed(sdtfe_tess *t, const MapGeom &g, int n, const double *x,
                         const double *y, const int *entry, const uint64_t *pixel,
                         const uint32_t *sample, double *value, int *crossed);
@@ -163,9 +164,16 @@ int ensure_map_buf(sdtfe_tess *t, size_t npix) {
   return 0;
 }
 
-int read_stats(sdtfe_tess *t, sdtfe_stats *s) {
-  unsigned long long c[ST_COUNT];
-  SDTFE_HIP(hipMemcpyAsync(c, t->stats, sizeof c, hipMemcpyDeviceToHost, t->stream));
+// Pinned host memory for a call's small read-backs (a pageable destination
+// costs the runtime a staged copy of its own): the first buffer of a pooled
+// staging, or null when the pool could not pin memory.
+void *pinned_words(const PooledStaging &ps) { return ps.st ? ps.st->buf[0] : nullptr; }
+
+// The counters, read through host (pinned, ST_COUNT words) when given.
+int read_stats(sdtfe_tess *t, sdtfe_stats *s, void *host = nullptr) {
+  unsigned long long local[ST_COUNT];
+  unsigned long long *c = host ? static_cast<unsigned long long *>(host) : local;
+  SDTFE_HIP(hipMemcpyAsync(c, t->stats, sizeof local, hipMemcpyDeviceToHost, t->stream));
   SDTFE_HIP(hipStreamSynchronize(t->stream));
   if (s) fill_stats_pub(c, s);
   return 0;
@@ -487,12 +495,12 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
 }
 
 int run_map(sdtfe_tess *t, const sdtfe_map_params *p, double *out, int *navg, int accumulate,
-            double *t_walk_ms, double *t_kernel_ms) {
+            double *t_walk_ms, double *t_kernel_ms, void *pinned) {
   const MapGeom g = make_geom_pub(t, *p);
   SDTFE_HIP(hipMemsetAsync(t->stats, 0, sizeof(unsigned long long) * ST_COUNT, t->stream));
   SDTFE_HIP(hipEventRecord(t->ev0, t->stream));
   if (p->precision == SDTFE_PREC_MIXED) {
-    const int rc = launch_map_mixed(t, g, out, navg, accumulate);
+    const int rc = launch_map_mixed(t, g, out, navg, accumulate, static_cast<int *>(pinned));
     if (rc) return rc;
   } else {
     launch_map_fp64(t, g, out, navg, accumulate, t->stats);
@@ -696,9 +704,10 @@ int sdtfe_map(sdtfe_tess *t, const sdtfe_map_params *p, double *rho, int *n_avg,
   const size_t npix = (size_t)p->grid_dim * (size_t)p->grid_dim;
   if ((rc = ensure_map_buf(t, npix))) return rc;
   double t_walk = 0, t_kernel = 0;
-  if ((rc = run_map(t, p, t->map_buf, t->navg_buf, 0, &t_walk, &t_kernel))) return rc;
   {
     PooledStaging ps;
+    void *words = pinned_words(ps);
+    if ((rc = run_map(t, p, t->map_buf, t->navg_buf, 0, &t_walk, &t_kernel, words))) return rc;
     rc = download(t->map_buf, sizeof(double) * npix, t->stream, ps.st,
                   [rho](const char *h, size_t off, size_t n) {
                     const double *v = reinterpret_cast<const double *>(h);
@@ -710,9 +719,9 @@ int sdtfe_map(sdtfe_tess *t, const sdtfe_map_params *p, double *rho, int *n_avg,
                     [n_avg](const char *h, size_t off, size_t n) {
                       std::memcpy(reinterpret_cast<char *>(n_avg) + off, h, n);
                     });
+    if (!rc) rc = read_stats(t, stats, words);  // the downloads have finished with words
     if (rc) return rc;
   }
-  if ((rc = read_stats(t, stats))) return rc;
   if (stats) {
     stats->t_walk_ms = t_walk;
     stats->t_kernel_ms = t_kernel;
@@ -729,10 +738,12 @@ int sdtfe_map_device(sdtfe_tess *t, const sdtfe_map_params *p, double *rho_devic
   if (!rho_device) return fail(SDTFE_EINVAL, "null rho");
   auto t0 = Clock::now();
   double t_walk = 0, t_kernel = 0;
-  if ((rc = run_map(t, p, rho_device, nullptr, 1, stats ? &t_walk : nullptr, &t_kernel)))
+  PooledStaging ps;
+  void *words = pinned_words(ps);
+  if ((rc = run_map(t, p, rho_device, nullptr, 1, stats ? &t_walk : nullptr, &t_kernel, words)))
     return rc;
   if (stats) {
-    if ((rc = read_stats(t, stats))) return rc;
+    if ((rc = read_stats(t, stats, words))) return rc;
     stats->t_walk_ms = t_walk;
     stats->t_kernel_ms = t_kernel;
     stats->t_total_ms = ms_since(t0);

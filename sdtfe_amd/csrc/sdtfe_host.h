@@ -190,6 +190,7 @@ struct MixedWork {
   double *val;                // per handed-over sample: value and whether it counts
   unsigned char *ok;
   size_t val_cap;
+  int *h_count = nullptr;     // pinned host word for the queue's length (null: pageable)
 };
 // Limits of sdtfe_map_device_noalloc: mixed maps of up to kNoAllocPixels
 // pixels; up to kNoAllocSamples handed-over samples take the per-sample

@@ -320,9 +320,11 @@ int defer_by_sample(sdtfe_tess *t, const MapGeom &g, const MixedConsts &mc, doub
   // (off[n] = the number of samples)
   defer_count_kernel<<<(n + 255) / 256, 256, 0, s>>>(w.q, n, g.samples, w.off);
   set_int_kernel<<<1, 1, 0, s>>>(w.off + n, 0);
-  int total = 0;
-  int rc = scan_exclusive(w.off, w.off, (int64_t)n + 1, w.tmp, &total, s);
+  int local = 0;
+  int *ht = w.h_count ? w.h_count : &local;
+  int rc = scan_exclusive(w.off, w.off, (int64_t)n + 1, w.tmp, ht, s);
   if (rc) return rc;
+  const int total = *ht;
   if (total <= 0) return fail(SDTFE_EHIP, "second pass: queued pixels list no samples");
   if ((size_t)total > w.val_cap) {
     if (!grow) {
@@ -415,10 +417,11 @@ int run_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int a
     const int max_wave = defer_wave_entries();
     launch_deferred_waves(t, g, mc, out, navg, accumulate, dq, max_wave, w.stats, wide);
     SDTFE_HIP(hipGetLastError());
-    int count = 0;
-    SDTFE_HIP(hipMemcpyAsync(&count, w.q_count, sizeof(int), hipMemcpyDeviceToHost,
-                             t->stream));
+    int local = 0;
+    int *hc = w.h_count ? w.h_count : &local;
+    SDTFE_HIP(hipMemcpyAsync(hc, w.q_count, sizeof(int), hipMemcpyDeviceToHost, t->stream));
     SDTFE_HIP(hipStreamSynchronize(t->stream));
+    const int count = *hc;
     if (count > dq.cap) {
       // overflow: the queued pixels beyond the queue were not written; the
       // repeat writes every pixel again
@@ -427,7 +430,9 @@ int run_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int a
       zero_u64_kernel<<<1, 64, 0, t->stream>>>(w.stats, ST_COUNT);
       int rc = ensure_defer(t, npix);
       if (rc) return rc;
+      int *const h_count = w.h_count;
       w = tess_work(t);
+      w.h_count = h_count;
       continue;
     }
     if (count > max_wave)
@@ -438,13 +443,15 @@ int run_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int a
 
 // The queue holds a quarter of the pixels (all of them for a map that
 // accumulates); a map that overflows it is run again with room for all.
-int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate) {
+int launch_map_mixed(sdtfe_tess *t, const MapGeom &g, double *out, int *navg, int accumulate,
+                     int *h_count) {
   const size_t npix = (size_t)g.grid * (size_t)g.grid;
   if (npix > (size_t)INT_MAX) return fail(SDTFE_EINVAL, "map too large for the mixed walk");
   const size_t want = accumulate ? npix : std::max(npix / 4, std::min<size_t>(npix, 65536));
   int rc = ensure_defer(t, want);
   if (rc) return rc;
   MixedWork w = tess_work(t);
+  w.h_count = h_count;
   return run_map_mixed(t, g, out, navg, accumulate, w, true);
 }
 
