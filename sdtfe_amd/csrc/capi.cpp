@@ -476,15 +476,13 @@ int create(const double *pd, int n, const int *tetra, int nt, float p_mass, bool
   if (!rc && dev_malloc((void **)&t->stats, sizeof(unsigned long long) * ST_COUNT, t->stream) !=
                  hipSuccess)
     rc = fail(SDTFE_ENOMEM, "stats allocation failed");
-  if (!rc) {
-    if (st) {
-      rc = build_field(t, pd, tetra, st);
-    } else {
-      PooledStaging ps;
-      rc = build_field(t, pd, tetra, ps.st);
-    }
+  std::unique_ptr<PooledStaging> pooled;  // when the caller brings no staging
+  if (!rc && !st) {
+    pooled = std::make_unique<PooledStaging>();
+    st = pooled->st;
   }
-  if (!rc) rc = build_entry_grid(t);
+  if (!rc) rc = build_field(t, pd, tetra, st);
+  if (!rc) rc = build_entry_grid(t, st);
   if (rc) {
     free_tess(t);
     return rc;

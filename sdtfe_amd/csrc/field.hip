@@ -475,23 +475,56 @@ int dev_alloc(sdtfe_tess *t, T **p, size_t count) {
   return 0;
 }
 
-// Temporaries of one setup, freed in stream order when it ends.
+// Temporaries of one set-up step, freed in stream order when it ends.  A
+// step that reserves the sum of its arrays' sizes (size()) first gets them
+// carved from one allocation: one allocation and one free instead of one per
+// array.  An array that does not fit what is left gets a block of its own.
 struct Scratch {
   hipStream_t s;
   std::vector<void *> ptrs;
+  char *cur = nullptr;
+  size_t left = 0;
   explicit Scratch(hipStream_t stream) : s(stream) {}
   ~Scratch() {
     for (void *p : ptrs) dev_free(p, s);
   }
   template <typename T>
+  static size_t size(size_t count) {
+    return (sizeof(T) * (count > 0 ? count : 1) + 255) & ~(size_t)255;
+  }
+  int reserve(size_t bytes) {
+    void *p = nullptr;
+    if (dev_malloc(&p, bytes, s) != hipSuccess)
+      return fail(SDTFE_ENOMEM, "device allocation (scratch) failed");
+    ptrs.push_back(p);
+    cur = static_cast<char *>(p);
+    left = bytes;
+    return 0;
+  }
+  template <typename T>
   int get(T **p, size_t count) {
-    size_t bytes = sizeof(T) * (count > 0 ? count : 1);
+    const size_t bytes = size<T>(count);
+    if (bytes <= left) {
+      *p = reinterpret_cast<T *>(cur);
+      cur += bytes;
+      left -= bytes;
+      return 0;
+    }
     hipError_t e = dev_malloc((void **)p, bytes, s);
     if (e != hipSuccess) return fail(SDTFE_ENOMEM, "device allocation (scratch) failed");
     ptrs.push_back(*p);
     return 0;
   }
 };
+
+// Destination of a small read-back: the first pinned buffer of st when it
+// has one (a pageable destination costs the runtime a staged copy of its
+// own), else local.  st's uploads were queued on the same stream before, so
+// the read-back lands after they have read the buffer.
+template <typename T>
+T *readback_words(Staging *st, T *local) {
+  return st && st->buf[0] ? static_cast<T *>(st->buf[0]) : local;
+}
 
 inline unsigned blocks_for(int64_t n, int threads) {
   return (unsigned)((n + threads - 1) / threads);
@@ -513,26 +546,29 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra, Staging *st) 
   SDTFE_RC(upload(t->pos4, pd, sizeof(double) * 4 * (size_t)n, s, st));
   SDTFE_RC(upload(t->tets, tetra, sizeof(int) * 8 * (size_t)nt, s, st));
 
+  // the index flag and the incidence counters and cursors start at zero:
+  // one array, one clear
   Scratch sc(s);
-  int *bad = nullptr;
-  SDTFE_RC(sc.get(&bad, 1));
-  SDTFE_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+  const size_t n1 = (size_t)n + 1;
+  const size_t tmp_ints = scan_tmp_ints((int64_t)std::max(n + 1, nt));
+  SDTFE_RC(sc.reserve(Scratch::size<int>(1 + 2 * n1) + Scratch::size<int>(n1) +
+                      Scratch::size<int>(4 * (size_t)nt) + Scratch::size<int>(tmp_ints)));
+  int *zeroed = nullptr, *off = nullptr, *inc = nullptr, *tmp = nullptr;
+  SDTFE_RC(sc.get(&zeroed, 1 + 2 * n1));
+  SDTFE_RC(sc.get(&off, n1));
+  SDTFE_RC(sc.get(&inc, 4 * (size_t)nt));
+  SDTFE_RC(sc.get(&tmp, tmp_ints));
+  int *const bad = zeroed, *const deg = zeroed + 1, *const cursor = zeroed + 1 + n1;
+  SDTFE_HIP(hipMemsetAsync(zeroed, 0, sizeof(int) * (1 + 2 * n1), s));
   if (nt > 0) validate_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(t->tets, nt, n, bad);
-  int bad_h = 0;
-  SDTFE_HIP(hipMemcpyAsync(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  int bad_local = 0;
+  int *bad_h = readback_words(st, &bad_local);
+  SDTFE_HIP(hipMemcpyAsync(bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
   SDTFE_HIP(hipStreamSynchronize(s));
-  if (bad_h) return fail(SDTFE_ETOPO, "tetra_data holds an index out of range");
+  if (*bad_h) return fail(SDTFE_ETOPO, "tetra_data holds an index out of range");
 
   // volumes and the vertex -> tet incidence (CSR)
   SDTFE_RC(dev_alloc(t, &t->vol, (size_t)nt));
-  int *deg = nullptr, *off = nullptr, *cursor = nullptr, *inc = nullptr, *tmp = nullptr;
-  SDTFE_RC(sc.get(&deg, (size_t)n + 1));
-  SDTFE_RC(sc.get(&off, (size_t)n + 1));
-  SDTFE_RC(sc.get(&cursor, (size_t)n + 1));
-  SDTFE_RC(sc.get(&inc, 4 * (size_t)nt));
-  SDTFE_RC(sc.get(&tmp, scan_tmp_ints((int64_t)std::max(n + 1, nt))));
-  SDTFE_HIP(hipMemsetAsync(deg, 0, sizeof(int) * ((size_t)n + 1), s));
-  SDTFE_HIP(hipMemsetAsync(cursor, 0, sizeof(int) * ((size_t)n + 1), s));
   if (nt > 0)
     volume_degree_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->vol, deg);
@@ -580,7 +616,7 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra, Staging *st) 
   return 0;
 }
 
-int build_entry_grid(sdtfe_tess *t) {
+int build_entry_grid(sdtfe_tess *t, Staging *st) {
   const int nt = t->nt;
   hipStream_t s = t->stream;
   Scratch sc(s);
@@ -588,6 +624,9 @@ int build_entry_grid(sdtfe_tess *t) {
   int *cnt = nullptr, *off = nullptr, *tmp = nullptr;
   const unsigned nblk = blocks_for(nt, 256);
   double *part = nullptr, *box = nullptr;
+  SDTFE_RC(sc.reserve(Scratch::size<unsigned char>((size_t)nt) +
+                      2 * Scratch::size<int>((size_t)nt) + Scratch::size<int>(scan_tmp_ints(nt)) +
+                      Scratch::size<double>(4 * (size_t)nblk) + Scratch::size<double>(5)));
   SDTFE_RC(sc.get(&mask, (size_t)nt));
   SDTFE_RC(sc.get(&cnt, (size_t)nt));
   SDTFE_RC(sc.get(&off, (size_t)nt));
@@ -603,8 +642,9 @@ int build_entry_grid(sdtfe_tess *t) {
     SDTFE_RC(scan_exclusive(cnt, off, nt, tmp, nullptr, s));
     hull_box_kernel<<<1, 256, 0, s>>>(part, (int)nblk, scan_total_device(tmp, nt), box);
     SDTFE_HIP(hipGetLastError());
-    double h[5];
-    SDTFE_HIP(hipMemcpyAsync(h, box, sizeof h, hipMemcpyDeviceToHost, s));
+    double local[5];
+    double *h = readback_words(st, local);
+    SDTFE_HIP(hipMemcpyAsync(h, box, sizeof local, hipMemcpyDeviceToHost, s));
     SDTFE_HIP(hipStreamSynchronize(s));
     n_tri = (int)h[4];
     if (n_tri > 0)
@@ -640,17 +680,21 @@ int build_entry_grid(sdtfe_tess *t) {
   g.sx = (g.x1 > g.x0) ? g.nx / (g.x1 - g.x0) : 0.0;
   g.sy = (g.y1 > g.y0) ? g.ny / (g.y1 - g.y0) : 0.0;
   const int n_cell = g.nx * g.ny;
-  int *ccnt = nullptr, *ccur = nullptr, *ctmp = nullptr;
-  SDTFE_RC(sc.get(&ccnt, (size_t)n_cell + 1));
-  SDTFE_RC(sc.get(&ccur, (size_t)n_cell + 1));
+  const size_t c1 = (size_t)n_cell + 1;
+  // per-cell counts and cursors start at zero: one array, one clear
+  int *ccnt = nullptr, *ctmp = nullptr;
+  SDTFE_RC(sc.reserve(Scratch::size<int>(2 * c1) + Scratch::size<int>(scan_tmp_ints(n_cell + 1))));
+  SDTFE_RC(sc.get(&ccnt, 2 * c1));
   SDTFE_RC(sc.get(&ctmp, scan_tmp_ints(n_cell + 1)));
-  SDTFE_RC(dev_alloc(t, &t->cell_off, (size_t)n_cell + 1));
-  SDTFE_HIP(hipMemsetAsync(ccnt, 0, sizeof(int) * ((size_t)n_cell + 1), s));
-  SDTFE_HIP(hipMemsetAsync(ccur, 0, sizeof(int) * ((size_t)n_cell + 1), s));
+  int *const ccur = ccnt + c1;
+  SDTFE_RC(dev_alloc(t, &t->cell_off, c1));
+  SDTFE_HIP(hipMemsetAsync(ccnt, 0, sizeof(int) * 2 * c1, s));
   if (n_tri > 0)
     cell_bin_kernel<false><<<blocks_for(n_tri, 256), 256, 0, s>>>(g, ccnt, nullptr, nullptr);
-  int n_entries = 0;
-  SDTFE_RC(scan_exclusive(ccnt, t->cell_off, (int64_t)n_cell + 1, ctmp, &n_entries, s));
+  int entries_local = 0;
+  int *entries_h = readback_words(st, &entries_local);
+  SDTFE_RC(scan_exclusive(ccnt, t->cell_off, (int64_t)n_cell + 1, ctmp, entries_h, s));
+  const int n_entries = *entries_h;
   SDTFE_RC(dev_alloc(t, &t->cell_tri, (size_t)n_entries));
   g.cell_off = t->cell_off;
   if (n_tri > 0)

@@ -162,7 +162,7 @@ bool wide_records(int64_t n_tetra);
 // field.hip
 int build_field(sdtfe_tess *t, const double *particle_data, const int *tetra_data,
                 Staging *st);
-int build_entry_grid(sdtfe_tess *t);
+int build_entry_grid(sdtfe_tess *t, Staging *st);
 // Forward-hull faces of host arrays, on the GPU (no tessellation handle):
 // mask[t] bit f marks face f of tet t (src/triangulate.c:274-311); the
 // faces a mask marks as vertex triples and tets in tet, then face order
