@@ -421,10 +421,10 @@ namespace {
 
 void free_tess(sdtfe_tess *t) {
   if (!t) return;
-  void *bufs[] = {t->pos4,    t->tets,     t->vol,     t->nabla,   t->rec64,
-                  t->rec32,   t->facerec,   t->frec64,   t->tri_xy,   t->tri_tet, t->tri_rid, t->tri_abc, t->cell_off,
-                  t->cell_tri, t->stats,   t->map_buf, t->navg_buf, t->defer_buf,
-                  t->defer_count, t->dq_off, t->dq_tmp, t->dq_val, t->dq_ok};
+  // pos4 .. frec64 and tri_xy .. tri_abc live in field_block and tri_block
+  void *bufs[] = {t->field_block, t->tri_block, t->cell_off, t->cell_tri,    t->stats,
+                  t->map_buf,     t->navg_buf,  t->defer_buf, t->defer_count, t->dq_off,
+                  t->dq_tmp,      t->dq_val,    t->dq_ok};
   if (t->stream) {
     // The frees are stream-ordered: work still queued on the stream ends
     // first.  A stream of the caller's pool is not drained, so freeing never

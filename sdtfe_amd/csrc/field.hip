@@ -464,15 +464,18 @@ __global__ void cell_bin_kernel(EntryGrid g, int *cnt_or_cursor, const int *off,
 
 namespace {
 
-template <typename T>
-int dev_alloc(sdtfe_tess *t, T **p, size_t count) {
-  size_t bytes = sizeof(T) * (count > 0 ? count : 1);
-  hipError_t e = dev_malloc((void **)p, bytes, t->stream);
+int dev_block(sdtfe_tess *t, void **p, size_t bytes) {
+  hipError_t e = dev_malloc(p, bytes, t->stream);
   if (e != hipSuccess) return fail(SDTFE_ENOMEM, "device allocation failed (" +
                                                      std::to_string(bytes) +
                                                      " B): " + hipGetErrorString(e));
   t->device_bytes += (int64_t)bytes;
   return 0;
+}
+
+template <typename T>
+int dev_alloc(sdtfe_tess *t, T **p, size_t count) {
+  return dev_block(t, (void **)p, sizeof(T) * (count > 0 ? count : 1));
 }
 
 // Temporaries of one set-up step, freed in stream order when it ends.  A
@@ -538,11 +541,44 @@ inline unsigned blocks_for(int64_t n, int threads) {
     if (_rc) return _rc;   \
   } while (0)
 
+namespace {
+
+// Consecutive arrays of one allocation, each on a 256-byte boundary; with a
+// null base it only adds up their sizes.
+struct Carve {
+  char *base = nullptr;
+  size_t off = 0;
+  template <typename T>
+  void take(T **p, size_t count) {
+    if (base) *p = reinterpret_cast<T *>(base + off);
+    off += Scratch::size<T>(count);
+  }
+};
+
+}  // namespace
+
 int build_field(sdtfe_tess *t, const double *pd, const int *tetra, Staging *st) {
   const int n = t->n, nt = t->nt;
   hipStream_t s = t->stream;
-  SDTFE_RC(dev_alloc(t, &t->pos4, 4 * (size_t)n));
-  SDTFE_RC(dev_alloc(t, &t->tets, 8 * (size_t)nt));
+  // the per-particle and per-tet arrays: one allocation, freed with the
+  // tessellation (the FP64 walk's face records are addressed as 4 x tet +
+  // face in an int, so they are built below 2^29 tets)
+  const bool frec64 = t->fp64_faces && nt > 0 && nt < (1 << 29);
+  auto layout = [&](Carve &c) {
+    c.take(&t->pos4, 4 * (size_t)n);
+    c.take(&t->tets, 8 * (size_t)nt);
+    c.take(&t->vol, (size_t)nt);
+    c.take(&t->nabla, 3 * (size_t)nt);
+    c.take(&t->rec64, (size_t)nt);
+    c.take(&t->rec32, (size_t)nt);
+    c.take(&t->facerec, 4 * (size_t)nt);
+    if (frec64) c.take(&t->frec64, 4 * (size_t)nt);
+  };
+  Carve sizing;
+  layout(sizing);
+  SDTFE_RC(dev_block(t, &t->field_block, sizing.off));
+  Carve carve{static_cast<char *>(t->field_block)};
+  layout(carve);
   SDTFE_RC(upload(t->pos4, pd, sizeof(double) * 4 * (size_t)n, s, st));
   SDTFE_RC(upload(t->tets, tetra, sizeof(int) * 8 * (size_t)nt, s, st));
 
@@ -568,7 +604,6 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra, Staging *st) 
   if (*bad_h) return fail(SDTFE_ETOPO, "tetra_data holds an index out of range");
 
   // volumes and the vertex -> tet incidence (CSR)
-  SDTFE_RC(dev_alloc(t, &t->vol, (size_t)nt));
   if (nt > 0)
     volume_degree_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->vol, deg);
@@ -595,22 +630,15 @@ int build_field(sdtfe_tess *t, const double *pd, const int *tetra, Staging *st) 
   t->z_min = n > 0 ? lo[2] : 0.0;
   t->z_max = n > 0 ? hi[2] : 0.0;
 
-  SDTFE_RC(dev_alloc(t, &t->nabla, 3 * (size_t)nt));
-  SDTFE_RC(dev_alloc(t, &t->rec64, (size_t)nt));
-  SDTFE_RC(dev_alloc(t, &t->rec32, (size_t)nt));
-  SDTFE_RC(dev_alloc(t, &t->facerec, 4 * (size_t)nt));
   if (nt > 0) {
     gradient_record_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->nabla, t->rec64, t->rec32);
     face_record_kernel<<<blocks_for(4 * (int64_t)nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, t->facerec);
   }
-  // face records of the FP64 walk, addressed as 4 x tet + face in an int
-  if (t->fp64_faces && nt > 0 && nt < (1 << 29)) {
-    SDTFE_RC(dev_alloc(t, &t->frec64, 4 * (size_t)nt));
+  if (frec64)
     face_record64_kernel<<<blocks_for(4 * (int64_t)nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, t->nabla, nt, t->frec64);
-  }
   // the entry grid follows on the same stream and ends with its sync
   SDTFE_HIP(hipGetLastError());
   return 0;
@@ -651,10 +679,17 @@ int build_entry_grid(sdtfe_tess *t, Staging *st) {
       for (int j = 0; j < 4; ++j) bb[j] = h[j];
   }
   t->n_tri = n_tri;
-  SDTFE_RC(dev_alloc(t, &t->tri_xy, 6 * (size_t)n_tri));
-  SDTFE_RC(dev_alloc(t, &t->tri_tet, (size_t)n_tri));
-  SDTFE_RC(dev_alloc(t, &t->tri_rid, (size_t)n_tri));
-  SDTFE_RC(dev_alloc(t, &t->tri_abc, 3 * (size_t)n_tri));
+  auto layout = [&](Carve &c) {
+    c.take(&t->tri_xy, 6 * (size_t)n_tri);
+    c.take(&t->tri_tet, (size_t)n_tri);
+    c.take(&t->tri_rid, (size_t)n_tri);
+    c.take(&t->tri_abc, 3 * (size_t)n_tri);
+  };
+  Carve sizing;
+  layout(sizing);
+  SDTFE_RC(dev_block(t, &t->tri_block, sizing.off));
+  Carve carve{static_cast<char *>(t->tri_block)};
+  layout(carve);
   if (nt > 0)
     emit_tri_kernel<<<blocks_for(nt, 256), 256, 0, s>>>(
         reinterpret_cast<const double4 *>(t->pos4), t->tets, nt, mask, off, t->tri_xy,

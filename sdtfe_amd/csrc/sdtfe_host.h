@@ -50,7 +50,9 @@ struct sdtfe_tess {
   bool own_stream = true;  // false: a stream of the caller's pool (the server's)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evk0 = nullptr, evk1 = nullptr;  // around the walk kernel proper
-  // particles and tets as uploaded, plus the field
+  // particles and tets as uploaded, plus the field; pos4 .. frec64 are
+  // arrays of field_block, one allocation
+  void *field_block = nullptr;
   double *pos4 = nullptr;     // N x 4 (x, y, z, rho)
   int *tets = nullptr;        // T x 8
   double *vol = nullptr;      // T
@@ -62,7 +64,8 @@ struct sdtfe_tess {
   double org[3] = {0, 0, 0};  // centre of the particles' bounding box
   double span = 0;            // its half-extent (max over axes)
   double z_min = 0, z_max = 0;  // the particles' (and the hull's) z range
-  // entry structure
+  // entry structure; tri_xy .. tri_abc are arrays of tri_block
+  void *tri_block = nullptr;
   double *tri_xy = nullptr;
   int *tri_tet = nullptr;
   int *tri_rid = nullptr;     // F: 4 x tet + face, with face records
